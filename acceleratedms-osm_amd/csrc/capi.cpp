@@ -1,0 +1,636 @@
+// C-ABI implementation: design wrappers, the device plan (MSOSM_GPU_BATCH device pipeline) and the
+// fold object (Fold_GPU). Host-side only; device code lives in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+#include "kernels.h"
+#include "msosm_amd.h"
+
+namespace msosm {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+} // namespace msosm
+
+using namespace msosm;
+
+#define HIPCHK(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess)                                                                                          \
+            return set_error(MSOSM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " (" + __FILE__ +      \
+                                                ":" + std::to_string(__LINE__) + ")");                                 \
+    } while (0)
+
+static int ilog2(long long v)
+{
+    int r = -1;
+    while (v) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+static bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+extern "C" const char *msosm_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int msosm_version(void) { return 100; }
+
+// ------------------------------------------------------------------------------------------------
+// design
+// ------------------------------------------------------------------------------------------------
+extern "C" int msosm_design_nd(float bw, float f0, float dm, int *nd)
+{
+    if (!nd || !(bw > 0) || !(f0 > 0) || !(dm >= 0)) return set_error(MSOSM_ERR_ARG, "design_nd: need bw>0, f0>0, dm>=0");
+    *nd = design_nd(bw, f0, dm);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_min_order(int nd, int *M)
+{
+    if (!M || nd < 0) return set_error(MSOSM_ERR_ARG, "design_min_order: nd must be >= 0");
+    *M = nd == 0 ? 0 : design_min_order(nd);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_osm_order(unsigned long nd, unsigned long *M)
+{
+    if (!M) return set_error(MSOSM_ERR_ARG, "design_osm_order: null output");
+    *M = design_osm_order(nd);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_delaycount(int nd, int M, int *dc)
+{
+    if (!dc || M <= 0 || nd < 0) return set_error(MSOSM_ERR_ARG, "design_delaycount: need M>0, nd>=0");
+    *dc = design_delaycount(nd, M);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_segmentation(float bw, float f0, float dm, int M, int *delay_points, int *delaycount)
+{
+    if (!delay_points || !is_pow2(M) || !(bw > 0) || !(f0 > 0) || !(dm >= 0))
+        return set_error(MSOSM_ERR_ARG, "design_segmentation: need M a power of two, bw>0, f0>0, dm>=0");
+    const int dc = design_segmentation(bw, f0, dm, M, delay_points);
+    for (long k = 0; k < 2L * M; ++k)
+        if (delay_points[k] < 0)
+            return set_error(MSOSM_ERR_ARG, "design_segmentation: bins left without a delay (inconsistent design)");
+    if (delaycount) *delaycount = dc;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_build_segments(const int *delay, int fft_len, int max_len, int *seg, int cap, int *count)
+{
+    if (!delay || !seg || !count || fft_len < 0 || max_len <= 0)
+        return set_error(MSOSM_ERR_ARG, "design_build_segments: bad arguments");
+    const int n = design_build_segments(delay, fft_len, max_len, seg, cap);
+    if (n < 0) return set_error(MSOSM_ERR_ARG, "design_build_segments: cap too small");
+    *count = n;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_design_transfer_function(float bw, float f0, float dm, int M, int phase_mode, float *H)
+{
+    if (!H || !is_pow2(M) || M < 2 || (phase_mode != MSOSM_PHASE_FP64 && phase_mode != MSOSM_PHASE_FP32))
+        return set_error(MSOSM_ERR_ARG, "design_transfer_function: need M a power of two >= 2 and a valid phase mode");
+    return design_transfer_function(bw, f0, dm, M, phase_mode, H, nullptr);
+}
+
+extern "C" int msosm_design_transfer_functions(float bw, float f0, const float *dms, int num_dms, int M,
+                                               int phase_mode, float *H, int nthreads)
+{
+    if (!H || !dms || num_dms <= 0 || !is_pow2(M) || M < 2 ||
+        (phase_mode != MSOSM_PHASE_FP64 && phase_mode != MSOSM_PHASE_FP32))
+        return set_error(MSOSM_ERR_ARG, "design_transfer_functions: bad arguments");
+    return design_transfer_functions(bw, f0, dms, num_dms, M, phase_mode, H, nthreads);
+}
+
+// ------------------------------------------------------------------------------------------------
+// fold object
+// ------------------------------------------------------------------------------------------------
+struct msosm_fold {
+    int device = 0;
+    double period = 0, fs = 0, dt = 0, t_start = 0;
+    unsigned long size = 0, nbins = 0;
+    int S = 1;
+    double *sums = nullptr;
+    unsigned long long *cnts = nullptr;
+    float *prof = nullptr;
+};
+
+static int fold_free(msosm_fold *f)
+{
+    if (!f) return MSOSM_OK;
+    (void)hipSetDevice(f->device);
+    if (f->sums) (void)hipFree(f->sums);
+    if (f->cnts) (void)hipFree(f->cnts);
+    if (f->prof) (void)hipFree(f->prof);
+    delete f;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_create(double period, double fs, unsigned long size, unsigned long nbins, int num_series,
+                                 int device, msosm_fold **out)
+{
+    if (!out || !(period > 0) || !(fs > 0) || size == 0 || num_series <= 0)
+        return set_error(MSOSM_ERR_ARG, "fold_create: need period>0, fs>0, size>0, num_series>0");
+    if (nbins == 0) {
+        // time_bin = -1 -> period_samples, computed as (unsigned long)(float period * float fs) (fold_gpu.cpp:10-15)
+        const float ps = (float)period * (float)fs;
+        nbins = (unsigned long)ps;
+        if (nbins == 0) return set_error(MSOSM_ERR_ARG, "fold_create: period*fs < 1 sample");
+    }
+    msosm_fold *f = new msosm_fold();
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+    }
+    f->device = device;
+    f->period = period;
+    f->fs = fs;
+    f->dt = 1.0 / fs; // fold_gpu.cpp:50
+    f->size = size;
+    f->nbins = nbins;
+    f->S = num_series;
+    const size_t n = nbins * (size_t)num_series;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipMalloc(&f->sums, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&f->cnts, n * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&f->prof, n * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(f->sums, 0, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(f->cnts, 0, n * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        fold_free(f);
+        return set_error(MSOSM_ERR_ALLOC, std::string("fold_create: ") + hipGetErrorString(e));
+    }
+    *out = f;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_destroy(msosm_fold *f) { return fold_free(f); }
+
+extern "C" int msosm_fold_get_nbins(msosm_fold *f, unsigned long *nbins)
+{
+    if (!f || !nbins) return set_error(MSOSM_ERR_ARG, "fold_get_nbins: null argument");
+    *nbins = f->nbins;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_accumulate(msosm_fold *f, const void *pol1, const void *pol2, long long stride, void *stream)
+{
+    if (!f || !pol1) return set_error(MSOSM_ERR_ARG, "fold_accumulate: null fold or series");
+    if (f->S > 1 && stride < (long long)f->size)
+        return set_error(MSOSM_ERR_ARG, "fold_accumulate: series stride shorter than the fold size");
+    HIPCHK(hipSetDevice(f->device));
+    HIPCHK(launch_fold_series(static_cast<const float2 *>(pol1), static_cast<const float2 *>(pol2),
+                              (long long)f->size, stride, f->S, f->nbins, f->t_start, f->dt, f->period, f->sums,
+                              f->cnts, static_cast<hipStream_t>(stream)));
+    f->t_start += static_cast<double>(f->size) * f->dt; // fold_gpu.cpp:53
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_read(msosm_fold *f, float *profile, double *sums, unsigned long long *counts)
+{
+    if (!f) return set_error(MSOSM_ERR_ARG, "fold_read: null fold");
+    HIPCHK(hipSetDevice(f->device));
+    const size_t n = f->nbins * (size_t)f->S;
+    HIPCHK(hipDeviceSynchronize());
+    if (profile) {
+        HIPCHK(launch_profile(f->sums, f->cnts, f->prof, (long long)n, nullptr));
+        HIPCHK(hipMemcpy(profile, f->prof, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (sums) HIPCHK(hipMemcpy(sums, f->sums, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (counts) HIPCHK(hipMemcpy(counts, f->cnts, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_reset(msosm_fold *f)
+{
+    if (!f) return set_error(MSOSM_ERR_ARG, "fold_reset: null fold");
+    HIPCHK(hipSetDevice(f->device));
+    HIPCHK(hipDeviceSynchronize());
+    const size_t n = f->nbins * (size_t)f->S;
+    HIPCHK(hipMemset(f->sums, 0, n * sizeof(double)));
+    HIPCHK(hipMemset(f->cnts, 0, n * sizeof(unsigned long long)));
+    f->t_start = 0.0;
+    return MSOSM_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// plan
+// ------------------------------------------------------------------------------------------------
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+struct msosm_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int M = 0, L = 0, log2L = 0, D = 0, B = 0, dc = 0, FBS = 0, in_fmt = 0, out_flags = 0, G = 0;
+    size_t in_bytes = 0;
+    void *in_dev = nullptr;
+    float2 *hist = nullptr, *ring = nullptr, *H = nullptr, *scratch = nullptr, *out_c64 = nullptr, *tw = nullptr;
+    uint16_t *delay = nullptr, *bins = nullptr;
+    uint32_t *out_u16 = nullptr;
+    int tw_n1 = 0, tw_n2 = 0, tw_lo = 0, tw_hi = 0, tw_s = 0;
+    long long g0 = 0;
+    msosm_fold *fold = nullptr;
+    size_t device_bytes = 0;
+    bool profile = false;
+    std::vector<EventPair> pending[2], pool;
+    double prof_ms[2] = {0, 0};
+    long long prof_n[2] = {0, 0};
+    std::vector<char> dm_set;
+};
+
+template <class T>
+static hipError_t dalloc(msosm_plan *p, T **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc((void **)ptr, bytes);
+    if (e == hipSuccess) p->device_bytes += bytes;
+    return e;
+}
+
+static void plan_free(msosm_plan *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    void *bufs[] = {p->in_dev, p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay, p->bins, p->out_u16};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (int k = 0; k < 2; ++k)
+        for (auto &e : p->pending[k]) p->pool.push_back(e);
+    for (auto &e : p->pool) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
+{
+    if (!d || !out) return set_error(MSOSM_ERR_ARG, "plan_create: null argument");
+    *out = nullptr;
+    if (!is_pow2(d->M)) return set_error(MSOSM_ERR_ARG, "plan_create: M must be a power of two");
+    const int log2L = ilog2(2LL * d->M);
+    if (!length_supported(log2L))
+        return set_error(MSOSM_ERR_UNSUPPORTED, "plan_create: FFT length 2M=" + std::to_string(2LL * d->M) +
+                                                     " outside the supported range [2^12, 2^22]");
+    if (d->num_dms <= 0 || d->batch <= 0) return set_error(MSOSM_ERR_ARG, "plan_create: num_dms and batch must be > 0");
+    if (d->delaycount <= 0 || d->delaycount > 65535)
+        return set_error(MSOSM_ERR_ARG, "plan_create: delaycount must be in [1, 65535]");
+    if (d->in_format < MSOSM_IN_U16 || d->in_format > MSOSM_IN_C64)
+        return set_error(MSOSM_ERR_ARG, "plan_create: unknown input format");
+    if (d->out_flags & ~(MSOSM_OUT_C64 | MSOSM_OUT_U16)) return set_error(MSOSM_ERR_ARG, "plan_create: unknown output flags");
+
+    msosm_plan *p = new msosm_plan();
+    int dev = d->device;
+    if (dev < 0) {
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    }
+    p->device = dev;
+    auto fail = [&](int code, const std::string &m) {
+        plan_free(p);
+        return set_error(code, m);
+    };
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: hipSetDevice: ") + hipGetErrorString(e));
+    p->M = d->M;
+    p->L = 2 * d->M;
+    p->log2L = log2L;
+    p->D = d->num_dms;
+    p->B = d->batch;
+    p->dc = d->delaycount;
+    p->in_fmt = d->in_format;
+    p->out_flags = d->out_flags;
+    // ring depth: roundup(delaycount + batch - 1, batch) (msosm_gpu_batch.cpp:113-117)
+    long long fbs = (long long)p->dc + p->B - 1;
+    if (fbs % p->B) fbs += p->B - fbs % p->B;
+    if (fbs > 0x7fffffffLL) return fail(MSOSM_ERR_ARG, "plan_create: ring too deep");
+    p->FBS = (int)fbs;
+    p->in_bytes = p->in_fmt == MSOSM_IN_U16 ? 4 : p->in_fmt == MSOSM_IN_I8 ? 2 : 8;
+
+    int cus = 0, max_lds = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: device query: ") + hipGetErrorString(e));
+    const int tiles_lds = tiles_lds_bytes(log2L);
+    const int inv_lds = tiles_lds + 2048 * 12;
+    if (inv_lds > max_lds)
+        return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(inv_lds) +
+                                               " B of LDS per workgroup, device allows " + std::to_string(max_lds));
+    e = configure_lds(log2L, tiles_lds, inv_lds);
+    if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: LDS attribute: ") + hipGetErrorString(e));
+    p->G = std::max(8, (cus / 8) * 8);
+
+    if (d->stream) {
+        p->stream = static_cast<hipStream_t>(d->stream);
+    } else {
+        e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: stream: ") + hipGetErrorString(e));
+        p->own_stream = true;
+    }
+
+    const size_t L = p->L, M = p->M, B = p->B, D = p->D;
+    // twiddle tables (FP64 -> complex64): n1, n2, lo, hi
+    int n1, n2;
+    split_length(log2L, &n1, &n2);
+    p->tw_s = (log2L + 1) / 2;
+    const int nlo = 1 << p->tw_s, nhi = (int)(L >> p->tw_s);
+    p->tw_n1 = 0;
+    p->tw_n2 = n1;
+    p->tw_lo = n1 + n2;
+    p->tw_hi = n1 + n2 + nlo;
+    const int ntw = n1 + n2 + nlo + nhi;
+    std::vector<float2> tw(ntw);
+    auto put = [&](int off, long k, long n) {
+        double re, im;
+        unit_root(k, n, &re, &im);
+        tw[off + k] = make_float2((float)re, (float)im);
+    };
+    for (int k = 0; k < n1; ++k) put(p->tw_n1, k, n1);
+    for (int k = 0; k < n2; ++k) put(p->tw_n2, k, n2);
+    for (int k = 0; k < nlo; ++k) put(p->tw_lo, k, (long)L);
+    for (int k = 0; k < nhi; ++k) {
+        double re, im;
+        unit_root((long)k << p->tw_s, (long)L, &re, &im);
+        tw[p->tw_hi + k] = make_float2((float)re, (float)im);
+    }
+
+    e = dalloc(p, &p->tw, ntw * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(p->tw, tw.data(), ntw * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(p, &p->in_dev, B * M * p->in_bytes);
+    if (e == hipSuccess) e = dalloc(p, &p->hist, M * sizeof(float2));
+    if (e == hipSuccess) e = dalloc(p, &p->ring, (size_t)p->FBS * L * sizeof(float2));
+    if (e == hipSuccess) e = dalloc(p, &p->H, D * L * sizeof(float2));
+    if (e == hipSuccess) e = dalloc(p, &p->delay, D * L * sizeof(uint16_t));
+    if (e == hipSuccess) e = dalloc(p, &p->scratch, (size_t)p->G * L * sizeof(float2));
+    if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
+    if (e == hipSuccess && (p->out_flags & MSOSM_OUT_U16)) e = dalloc(p, &p->out_u16, D * B * M * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(p->hist, 0, M * sizeof(float2));
+    if (e == hipSuccess) e = hipMemset(p->ring, 0, (size_t)p->FBS * L * sizeof(float2));
+    if (e == hipSuccess) e = hipMemset(p->H, 0, D * L * sizeof(float2));
+    if (e == hipSuccess) e = hipMemset(p->delay, 0, D * L * sizeof(uint16_t));
+    if (e != hipSuccess)
+        return fail(MSOSM_ERR_ALLOC, std::string("plan_create: device allocation: ") + hipGetErrorString(e));
+    p->dm_set.assign(D, 0);
+    *out = p;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_destroy(msosm_plan *p)
+{
+    plan_free(p);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, const float *H)
+{
+    if (!p || !delay_points || !H) return set_error(MSOSM_ERR_ARG, "plan_set_dm: null argument");
+    if (i < 0 || i >= p->D) return set_error(MSOSM_ERR_ARG, "plan_set_dm: dm index out of range");
+    std::vector<uint16_t> dl(p->L);
+    for (int k = 0; k < p->L; ++k) {
+        const int v = delay_points[k];
+        if (v < 0 || v >= p->dc)
+            return set_error(MSOSM_ERR_ARG, "plan_set_dm: delay_points[" + std::to_string(k) + "]=" + std::to_string(v) +
+                                                " outside [0, delaycount=" + std::to_string(p->dc) + ")");
+        dl[k] = (uint16_t)v;
+    }
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipMemcpy(p->delay + (size_t)i * p->L, dl.data(), p->L * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->H + (size_t)i * p->L, H, p->L * sizeof(float2), hipMemcpyHostToDevice));
+    p->dm_set[i] = 1;
+    return MSOSM_OK;
+}
+
+static int record_begin(msosm_plan *p, EventPair &ev)
+{
+    if (p->pool.empty()) {
+        EventPair e;
+        HIPCHK(hipEventCreate(&e.a));
+        HIPCHK(hipEventCreate(&e.b));
+        p->pool.push_back(e);
+    }
+    ev = p->pool.back();
+    p->pool.pop_back();
+    HIPCHK(hipEventRecord(ev.a, p->stream));
+    return MSOSM_OK;
+}
+
+static int collect_profile(msosm_plan *p)
+{
+    for (int k = 0; k < 2; ++k) {
+        for (auto &e : p->pending[k]) {
+            HIPCHK(hipEventSynchronize(e.b));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            p->prof_ms[k] += ms;
+            p->prof_n[k] += 1;
+            p->pool.push_back(e);
+        }
+        p->pending[k].clear();
+    }
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int on_device)
+{
+    if (!p || !input) return set_error(MSOSM_ERR_ARG, "plan_push: null argument");
+    for (int i = 0; i < p->D; ++i)
+        if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
+    if (!p->fold && !(p->out_flags & (MSOSM_OUT_C64 | MSOSM_OUT_U16)))
+        return set_error(MSOSM_ERR_STATE, "plan_push: no output selected and no fold attached");
+    HIPCHK(hipSetDevice(p->device));
+    const size_t nsamp = (size_t)p->B * p->M;
+    const void *src = input;
+    if (!on_device) {
+        HIPCHK(hipMemcpyAsync(p->in_dev, input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
+        src = p->in_dev;
+    }
+    const float2 *tw = p->tw;
+    FwdArgs fa{};
+    fa.in = src;
+    fa.in_fmt = p->in_fmt;
+    fa.hist = p->hist;
+    fa.ring = p->ring;
+    fa.FBS = p->FBS;
+    fa.g0 = p->g0;
+    fa.B = p->B;
+    fa.scratch = p->scratch;
+    fa.tw_n1 = tw + p->tw_n1;
+    fa.tw_n2 = tw + p->tw_n2;
+    fa.tw_lo = tw + p->tw_lo;
+    fa.tw_hi = tw + p->tw_hi;
+    fa.tw_s = p->tw_s;
+    EventPair ev{};
+    if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
+    HIPCHK(launch_fwd(p->log2L, fa, std::min(p->B, p->G), p->stream));
+    if (p->profile) {
+        HIPCHK(hipEventRecord(ev.b, p->stream));
+        p->pending[0].push_back(ev);
+    }
+    HIPCHK(launch_hist(p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M, p->stream));
+    InvArgs ia{};
+    ia.ring = p->ring;
+    ia.FBS = p->FBS;
+    ia.g0 = p->g0;
+    ia.B = p->B;
+    ia.D = p->D;
+    ia.H = p->H;
+    ia.delay = p->delay;
+    ia.scratch = p->scratch;
+    ia.out_c64 = p->out_c64;
+    ia.out_u16 = p->out_u16;
+    ia.tw_n1 = fa.tw_n1;
+    ia.tw_n2 = fa.tw_n2;
+    ia.tw_lo = fa.tw_lo;
+    ia.tw_hi = fa.tw_hi;
+    ia.tw_s = p->tw_s;
+    if (p->fold) {
+        msosm_fold *f = p->fold;
+        HIPCHK(launch_bins(p->bins, (long long)nsamp, f->nbins, f->t_start, f->dt, f->period, p->stream));
+        ia.bins = p->bins;
+        ia.fold_sum = f->sums;
+        ia.fold_cnt = f->cnts;
+        ia.nbins = (int)f->nbins;
+    }
+    if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
+    HIPCHK(launch_inv(p->log2L, ia, p->G, p->stream));
+    if (p->profile) {
+        HIPCHK(hipEventRecord(ev.b, p->stream));
+        p->pending[1].push_back(ev);
+    }
+    p->g0 += p->B;
+    if (p->fold) p->fold->t_start += static_cast<double>(nsamp) * p->fold->dt; // fold_gpu.cpp:53
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_output_c64(msosm_plan *p, void **ptr)
+{
+    if (!p || !ptr) return set_error(MSOSM_ERR_ARG, "plan_output_c64: null argument");
+    if (!p->out_c64) return set_error(MSOSM_ERR_STATE, "plan_output_c64: plan has no complex64 output");
+    *ptr = p->out_c64;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_output_u16(msosm_plan *p, void **ptr)
+{
+    if (!p || !ptr) return set_error(MSOSM_ERR_ARG, "plan_output_u16: null argument");
+    if (!p->out_u16) return set_error(MSOSM_ERR_STATE, "plan_output_u16: plan has no uint16 output");
+    *ptr = p->out_u16;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_get_output(msosm_plan *p, int which, void *host)
+{
+    if (!p || !host) return set_error(MSOSM_ERR_ARG, "plan_get_output: null argument");
+    const size_t n = (size_t)p->D * p->B * p->M;
+    HIPCHK(hipSetDevice(p->device));
+    if (which == MSOSM_OUT_C64) {
+        if (!p->out_c64) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no complex64 output");
+        HIPCHK(hipMemcpyAsync(host, p->out_c64, n * sizeof(float2), hipMemcpyDeviceToHost, p->stream));
+    } else if (which == MSOSM_OUT_U16) {
+        if (!p->out_u16) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no uint16 output");
+        HIPCHK(hipMemcpyAsync(host, p->out_u16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+    } else {
+        return set_error(MSOSM_ERR_ARG, "plan_get_output: which must be MSOSM_OUT_C64 or MSOSM_OUT_U16");
+    }
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_synchronize(msosm_plan *p)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_synchronize: null plan");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_reset(msosm_plan *p)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_reset: null plan");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(launch_fill(p->ring, (long long)p->FBS * p->L * sizeof(float2), 0u, p->stream));
+    HIPCHK(launch_fill(p->hist, (long long)p->M * sizeof(float2), 0u, p->stream));
+    p->g0 = 0;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_get_stream(msosm_plan *p, void **stream)
+{
+    if (!p || !stream) return set_error(MSOSM_ERR_ARG, "plan_get_stream: null argument");
+    *stream = p->stream;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_get_info(msosm_plan *p, msosm_plan_info *info)
+{
+    if (!p || !info) return set_error(MSOSM_ERR_ARG, "plan_get_info: null argument");
+    info->L = p->L;
+    info->FBS = p->FBS;
+    info->workgroups = p->G;
+    info->blocks_pushed = p->g0;
+    info->device_bytes = p->device_bytes;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_profile(msosm_plan *p, int enable)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_profile: null plan");
+    if (collect_profile(p) != MSOSM_OK) return MSOSM_ERR_HIP;
+    p->profile = enable != 0;
+    p->prof_ms[0] = p->prof_ms[1] = 0;
+    p->prof_n[0] = p->prof_n[1] = 0;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_profile_read(msosm_plan *p, int kernel, double *total_ms, long long *launches)
+{
+    if (!p || kernel < 0 || kernel > 1) return set_error(MSOSM_ERR_ARG, "plan_profile_read: bad arguments");
+    HIPCHK(hipSetDevice(p->device));
+    if (collect_profile(p) != MSOSM_OK) return MSOSM_ERR_HIP;
+    if (total_ms) *total_ms = p->prof_ms[kernel];
+    if (launches) *launches = p->prof_n[kernel];
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_attach_fold(msosm_plan *p, msosm_fold *f)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_attach_fold: null plan");
+    HIPCHK(hipSetDevice(p->device));
+    if (!f) {
+        p->fold = nullptr;
+        return MSOSM_OK;
+    }
+    const size_t nsamp = (size_t)p->B * p->M;
+    if (f->size != nsamp)
+        return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold size " + std::to_string(f->size) +
+                                            " != batch*M = " + std::to_string(nsamp));
+    if (f->S != p->D) return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold needs one series per DM");
+    if (f->nbins > 2048) return set_error(MSOSM_ERR_UNSUPPORTED, "plan_attach_fold: fused fold supports nbins <= 2048");
+    if (f->device != p->device) return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold and plan on different devices");
+    if (!p->bins) {
+        hipError_t e = dalloc(p, &p->bins, nsamp * sizeof(uint16_t));
+        if (e != hipSuccess) return set_error(MSOSM_ERR_ALLOC, std::string("plan_attach_fold: ") + hipGetErrorString(e));
+    }
+    p->fold = f;
+    return MSOSM_OK;
+}

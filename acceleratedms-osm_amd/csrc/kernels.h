@@ -1,0 +1,59 @@
+// Kernel argument blocks and launchers of the MS-OSM HIP path (device code in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "msosm_amd.h"
+
+namespace msosm {
+
+struct FwdArgs {
+    const void *in;        // this push's B*M input samples (device)
+    int in_fmt;            // MSOSM_IN_*
+    const float2 *hist;    // the M samples preceding `in` as complex64 (zeros before the stream start)
+    float2 *ring;          // [FBS][L] spectra
+    int FBS;
+    long long g0;          // stream block index of this push's first block
+    int B;                 // blocks per push
+    float2 *scratch;       // [grid][L]
+    const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
+    int tw_s;
+};
+
+struct InvArgs {
+    const float2 *ring;
+    int FBS;
+    long long g0;
+    int B, D;
+    const float2 *H;          // [D][L] transfer functions
+    const uint16_t *delay;    // [D][L] per-bin delay in blocks
+    float2 *scratch;          // [grid][L]
+    float2 *out_c64;          // [D][B][M] or null
+    uint32_t *out_u16;        // [D][B][M] or null
+    const uint16_t *bins;     // [B*M] fold bin of each call-relative sample, or null
+    double *fold_sum;         // [D][nbins] or null (no fold)
+    unsigned long long *fold_cnt;
+    int nbins;
+    const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
+    int tw_s;
+};
+
+void split_length(int log2L, int *n1, int *n2);
+bool length_supported(int log2L);
+int tiles_lds_bytes(int log2L);
+int workgroup_threads();
+hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes);
+hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st);
+hipError_t launch_inv(int log2L, const InvArgs &a, int grid, hipStream_t st);
+hipError_t launch_hist(float2 *hist, const void *in, int fmt, long long first, int M, hipStream_t st);
+hipError_t launch_bins(uint16_t *bins, long long n, unsigned long nbins, double t_start, double dt, double period,
+                       hipStream_t st);
+hipError_t launch_fold_series(const float2 *a, const float2 *b, long long size, long long stride, int nseries,
+                              unsigned long nbins, double t_start, double dt, double period, double *sums,
+                              unsigned long long *cnts, hipStream_t st);
+hipError_t launch_profile(const double *sums, const unsigned long long *cnts, float *prof, long long n,
+                          hipStream_t st);
+hipError_t launch_fill(void *p, long long nbytes, uint32_t value, hipStream_t st);
+
+} // namespace msosm

@@ -1,0 +1,339 @@
+"""Python front-end of the MI355X MS-OSM library (ctypes over the C ABI in include/msosm_amd.h).
+
+The compute path is the HIP library ``_lib/libmsosm_amd.so``; nothing here computes samples on the
+CPU. Importing this module does not touch the GPU; creating a :class:`Plan` does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_HERE, "_lib", "libmsosm_amd.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ALLOC = range(6)
+IN_U16, IN_I8, IN_C64 = 0, 1, 2
+OUT_NONE, OUT_C64, OUT_U16 = 0, 1, 2
+PHASE_FP64, PHASE_FP32 = 0, 1
+IN_BYTES = {IN_U16: 4, IN_I8: 2, IN_C64: 8}
+
+# every symbol declared in include/msosm_amd.h
+EXPORTS = [
+    "msosm_last_error", "msosm_version",
+    "msosm_design_nd", "msosm_design_min_order", "msosm_design_osm_order", "msosm_design_delaycount",
+    "msosm_design_segmentation", "msosm_design_build_segments", "msosm_design_transfer_function",
+    "msosm_design_transfer_functions",
+    "msosm_plan_create", "msosm_plan_destroy", "msosm_plan_set_dm", "msosm_plan_push", "msosm_plan_output_c64",
+    "msosm_plan_output_u16", "msosm_plan_get_output", "msosm_plan_synchronize", "msosm_plan_reset",
+    "msosm_plan_get_stream", "msosm_plan_get_info", "msosm_plan_profile", "msosm_plan_profile_read",
+    "msosm_fold_create", "msosm_fold_destroy", "msosm_fold_accumulate", "msosm_fold_read", "msosm_fold_reset",
+    "msosm_fold_get_nbins", "msosm_plan_attach_fold",
+]
+
+
+class MsosmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("M", C.c_int), ("num_dms", C.c_int), ("batch", C.c_int), ("delaycount", C.c_int),
+                ("in_format", C.c_int), ("out_flags", C.c_int), ("device", C.c_int), ("stream", C.c_void_p)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("L", C.c_int), ("FBS", C.c_int), ("workgroups", C.c_int), ("blocks_pushed", C.c_longlong),
+                ("device_bytes", C.c_size_t)]
+
+
+_lib = None
+
+
+def build(jobs: int = 8) -> None:
+    """Compile the HIP library in-tree (hipcc --offload-arch=gfx950)."""
+    subprocess.run(["make", "-s", "-C", PKG_DIR, f"-j{jobs}"], check=True)
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built (there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C {PKG_DIR}")
+    L = C.CDLL(LIB_PATH)
+    ip, fp, vp, dp = C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_void_p, C.POINTER(C.c_double)
+    sig = {
+        "msosm_last_error": ([], C.c_char_p),
+        "msosm_version": ([], C.c_int),
+        "msosm_design_nd": ([C.c_float, C.c_float, C.c_float, ip], C.c_int),
+        "msosm_design_min_order": ([C.c_int, ip], C.c_int),
+        "msosm_design_osm_order": ([C.c_ulong, C.POINTER(C.c_ulong)], C.c_int),
+        "msosm_design_delaycount": ([C.c_int, C.c_int, ip], C.c_int),
+        "msosm_design_segmentation": ([C.c_float, C.c_float, C.c_float, C.c_int, ip, ip], C.c_int),
+        "msosm_design_build_segments": ([ip, C.c_int, C.c_int, ip, C.c_int, ip], C.c_int),
+        "msosm_design_transfer_function": ([C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, fp], C.c_int),
+        "msosm_design_transfer_functions": ([C.c_float, C.c_float, fp, C.c_int, C.c_int, C.c_int, fp, C.c_int],
+                                            C.c_int),
+        "msosm_plan_create": ([C.POINTER(PlanDesc), C.POINTER(vp)], C.c_int),
+        "msosm_plan_destroy": ([vp], C.c_int),
+        "msosm_plan_set_dm": ([vp, C.c_int, ip, fp], C.c_int),
+        "msosm_plan_push": ([vp, vp, C.c_int], C.c_int),
+        "msosm_plan_output_c64": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_plan_output_u16": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_plan_get_output": ([vp, C.c_int, vp], C.c_int),
+        "msosm_plan_synchronize": ([vp], C.c_int),
+        "msosm_plan_reset": ([vp], C.c_int),
+        "msosm_plan_get_stream": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_plan_get_info": ([vp, C.POINTER(PlanInfo)], C.c_int),
+        "msosm_plan_profile": ([vp, C.c_int], C.c_int),
+        "msosm_plan_profile_read": ([vp, C.c_int, dp, C.POINTER(C.c_longlong)], C.c_int),
+        "msosm_fold_create": ([C.c_double, C.c_double, C.c_ulong, C.c_ulong, C.c_int, C.c_int, C.POINTER(vp)],
+                              C.c_int),
+        "msosm_fold_destroy": ([vp], C.c_int),
+        "msosm_fold_accumulate": ([vp, vp, vp, C.c_longlong, vp], C.c_int),
+        "msosm_fold_read": ([vp, fp, dp, C.POINTER(C.c_ulonglong)], C.c_int),
+        "msosm_fold_reset": ([vp], C.c_int),
+        "msosm_fold_get_nbins": ([vp, C.POINTER(C.c_ulong)], C.c_int),
+        "msosm_plan_attach_fold": ([vp, vp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise MsosmError(rc, lib().msosm_last_error().decode())
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ------------------------------------------------------------------------------------------------
+# design
+# ------------------------------------------------------------------------------------------------
+def design_nd(bw: float, f0: float, dm: float) -> int:
+    v = C.c_int()
+    _check(lib().msosm_design_nd(bw, f0, dm, C.byref(v)))
+    return v.value
+
+
+def design_min_order(nd: int) -> int:
+    v = C.c_int()
+    _check(lib().msosm_design_min_order(nd, C.byref(v)))
+    return v.value
+
+
+def design_osm_order(nd: int) -> int:
+    v = C.c_ulong()
+    _check(lib().msosm_design_osm_order(nd, C.byref(v)))
+    return v.value
+
+
+def design_delaycount(nd: int, M: int) -> int:
+    v = C.c_int()
+    _check(lib().msosm_design_delaycount(nd, M, C.byref(v)))
+    return v.value
+
+
+def design_segmentation(bw: float, f0: float, dm: float, M: int):
+    d = np.empty(2 * M, dtype=np.int32)
+    dc = C.c_int()
+    _check(lib().msosm_design_segmentation(bw, f0, dm, M, _ip(d), C.byref(dc)))
+    return d, dc.value
+
+
+def design_build_segments(delay: np.ndarray, max_len: int = 256) -> np.ndarray:
+    d = np.ascontiguousarray(delay, dtype=np.int32)
+    cap = d.size + 1
+    seg = np.empty(3 * cap, dtype=np.int32)
+    n = C.c_int()
+    _check(lib().msosm_design_build_segments(_ip(d), d.size, max_len, _ip(seg), cap, C.byref(n)))
+    return seg[: 3 * n.value].reshape(n.value, 3)
+
+
+def design_transfer_function(bw: float, f0: float, dm: float, M: int, phase_mode: int = PHASE_FP64) -> np.ndarray:
+    H = np.empty(2 * M, dtype=np.complex64)
+    _check(lib().msosm_design_transfer_function(bw, f0, dm, M, phase_mode, _fp(H.view(np.float32))))
+    return H
+
+
+def design_transfer_functions(bw: float, f0: float, dms, M: int, phase_mode: int = PHASE_FP64,
+                              nthreads: int = 0) -> np.ndarray:
+    d = np.ascontiguousarray(dms, dtype=np.float32)
+    H = np.empty((d.size, 2 * M), dtype=np.complex64)
+    _check(lib().msosm_design_transfer_functions(bw, f0, _fp(d), d.size, M, phase_mode, _fp(H.view(np.float32)),
+                                                 nthreads))
+    return H
+
+
+class Design:
+    """Design tables for a DM list at one M (Prepare_MSOSM, msosm_prepare.cpp:3-164,215-284)."""
+
+    def __init__(self, bw: float, f0: float, dms, fftpoint: int = 0, phase_mode: int = PHASE_FP64,
+                 M: int | None = None):
+        self.bw, self.f0 = float(bw), float(f0)
+        self.dms = np.ascontiguousarray(dms, dtype=np.float32)
+        self.nd = np.array([design_nd(bw, f0, float(dm)) for dm in self.dms], dtype=np.int64)
+        self.M_values = np.array([design_min_order(int(n)) for n in self.nd], dtype=np.int64)
+        self.M_min = int(self.M_values.max())
+        if M is None:
+            M = self.M_min
+            if fftpoint:
+                M = max(fftpoint // 2, self.M_min)
+        self.M = int(M)
+        segs = [design_segmentation(bw, f0, float(dm), self.M) for dm in self.dms]
+        self.delays = np.stack([s[0] for s in segs])
+        self.delaycount_values = np.array([s[1] for s in segs], dtype=np.int64)
+        self.delaycount = int(self.delaycount_values.max())
+        self.H = design_transfer_functions(bw, f0, self.dms, self.M, phase_mode)
+
+
+# ------------------------------------------------------------------------------------------------
+# device objects
+# ------------------------------------------------------------------------------------------------
+class Fold:
+    """Fold_GPU (folding/fold_gpu.cpp) over num_series series."""
+
+    def __init__(self, period: float, fs: float, size: int, nbins: int, num_series: int = 1, device: int = -1):
+        h = C.c_void_p()
+        _check(lib().msosm_fold_create(period, fs, size, nbins, num_series, device, C.byref(h)))
+        self.h = h
+        self.num_series = num_series
+        nb = C.c_ulong()
+        _check(lib().msosm_fold_get_nbins(h, C.byref(nb)))
+        self.nbins = nb.value
+
+    def accumulate(self, pol1_dev: int, pol2_dev: int | None = None, stride: int = 0, stream: int | None = None):
+        _check(lib().msosm_fold_accumulate(self.h, C.c_void_p(pol1_dev), C.c_void_p(pol2_dev) if pol2_dev else None,
+                                           stride, C.c_void_p(stream) if stream else None))
+
+    def read(self):
+        n = self.nbins * self.num_series
+        prof = np.empty(n, dtype=np.float32)
+        sums = np.empty(n, dtype=np.float64)
+        cnts = np.empty(n, dtype=np.uint64)
+        _check(lib().msosm_fold_read(self.h, _fp(prof), sums.ctypes.data_as(C.POINTER(C.c_double)),
+                                     cnts.ctypes.data_as(C.POINTER(C.c_ulonglong))))
+        shape = (self.num_series, self.nbins)
+        return prof.reshape(shape), sums.reshape(shape), cnts.reshape(shape)
+
+    def reset(self):
+        _check(lib().msosm_fold_reset(self.h))
+
+    def close(self):
+        if self.h:
+            lib().msosm_fold_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """The MSOSM_GPU_BATCH device pipeline (msosm_gpu_batch.cpp:12-245) behind the C ABI."""
+
+    def __init__(self, M: int, num_dms: int, batch: int, delaycount: int, in_format: int = IN_C64,
+                 out_flags: int = OUT_C64, device: int = -1, stream: int | None = None):
+        desc = PlanDesc(M, num_dms, batch, delaycount, in_format, out_flags, device, stream)
+        h = C.c_void_p()
+        _check(lib().msosm_plan_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        self.M, self.D, self.B, self.in_format, self.out_flags = M, num_dms, batch, in_format, out_flags
+        self.fold = None
+
+    @classmethod
+    def from_design(cls, design: Design, batch: int, in_format: int = IN_C64, out_flags: int = OUT_C64, **kw):
+        p = cls(design.M, len(design.dms), batch, design.delaycount, in_format, out_flags, **kw)
+        for i in range(len(design.dms)):
+            p.set_dm(i, design.delays[i], design.H[i])
+        return p
+
+    def set_dm(self, i: int, delay_points: np.ndarray, H: np.ndarray):
+        d = np.ascontiguousarray(delay_points, dtype=np.int32)
+        h = np.ascontiguousarray(H, dtype=np.complex64)
+        assert d.size == 2 * self.M and h.size == 2 * self.M
+        _check(lib().msosm_plan_set_dm(self.h, i, _ip(d), _fp(h.view(np.float32))))
+
+    def push(self, x) -> None:
+        """Push batch*M samples: a numpy array (host) or an int device address."""
+        if isinstance(x, int):
+            _check(lib().msosm_plan_push(self.h, C.c_void_p(x), 1))
+            return
+        a = np.ascontiguousarray(x)
+        assert a.nbytes == self.B * self.M * IN_BYTES[self.in_format], "push: wrong input size"
+        self._keep = a  # host buffer must outlive the asynchronous copy
+        _check(lib().msosm_plan_push(self.h, C.c_void_p(a.ctypes.data), 0))
+
+    def output_c64(self) -> np.ndarray:
+        out = np.empty((self.D, self.B * self.M), dtype=np.complex64)
+        _check(lib().msosm_plan_get_output(self.h, OUT_C64, C.c_void_p(out.ctypes.data)))
+        self.synchronize()
+        return out
+
+    def output_u16(self) -> np.ndarray:
+        out = np.empty((self.D, self.B * self.M), dtype=np.uint32)
+        _check(lib().msosm_plan_get_output(self.h, OUT_U16, C.c_void_p(out.ctypes.data)))
+        self.synchronize()
+        return out
+
+    def device_output_c64(self) -> int:
+        p = C.c_void_p()
+        _check(lib().msosm_plan_output_c64(self.h, C.byref(p)))
+        return p.value
+
+    def attach_fold(self, fold: Fold | None):
+        _check(lib().msosm_plan_attach_fold(self.h, fold.h if fold else None))
+        self.fold = fold
+
+    def synchronize(self):
+        _check(lib().msosm_plan_synchronize(self.h))
+
+    def reset(self):
+        _check(lib().msosm_plan_reset(self.h))
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().msosm_plan_get_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def info(self) -> PlanInfo:
+        i = PlanInfo()
+        _check(lib().msosm_plan_get_info(self.h, C.byref(i)))
+        return i
+
+    def profile(self, enable: bool = True):
+        _check(lib().msosm_plan_profile(self.h, int(enable)))
+
+    def profile_read(self, kernel: int):
+        ms, n = C.c_double(), C.c_longlong()
+        _check(lib().msosm_plan_profile_read(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().msosm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,161 @@
+/*
+ * msosm_amd.h -- C ABI of the MI355X-native MS-OSM coherent-dedispersion path.
+ *
+ * Plain pointers and sizes only. Host C++ (the reference-compatible classes in
+ * acceleratedms-osm_amd/csrc/reference_api/) and any FFI (ctypes, cgo, JNI) bind to these symbols.
+ * Citations name the reference interface each entry point replaces, as path:line relative to the
+ * reference repository root.
+ *
+ * Conventions
+ *  - Every function returns an int status (MSOSM_OK = 0); msosm_last_error() describes the last
+ *    failure of the calling thread.
+ *  - Complex values are interleaved float pairs (re, im), layout-compatible with cufftComplex,
+ *    hipFloatComplex and float2.
+ *  - "_dev" pointers are device (HBM) addresses; other buffers are host memory.
+ *  - A plan is externally synchronised: one host thread drives it; all of its GPU work is ordered
+ *    on the plan's stream.
+ */
+#ifndef MSOSM_AMD_H
+#define MSOSM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSOSM_OK = 0,
+    MSOSM_ERR_ARG = 1,         /* invalid argument */
+    MSOSM_ERR_HIP = 2,         /* HIP runtime failure */
+    MSOSM_ERR_STATE = 3,       /* call not valid in the object's current mode */
+    MSOSM_ERR_UNSUPPORTED = 4, /* valid request this build does not implement (e.g. FFT length) */
+    MSOSM_ERR_ALLOC = 5        /* host or device allocation failed */
+};
+
+/* Input sample formats */
+enum {
+    MSOSM_IN_U16 = 0, /* uint16 pair, offset binary, code 0 reserved for 0 (utils/include/globals.h:22-26) */
+    MSOSM_IN_I8 = 1,  /* int8 I/Q pair */
+    MSOSM_IN_C64 = 2  /* complex64 */
+};
+
+/* Output series selection (bit flags) */
+enum {
+    MSOSM_OUT_NONE = 0,
+    MSOSM_OUT_C64 = 1, /* complex64 [D][B][M] (fold mode buffer, msosm_gpu_batch.cpp:141-145) */
+    MSOSM_OUT_U16 = 2  /* uint16-pair [D][B][M] (save mode buffer, msosm_gpu_batch.cpp:146-150) */
+};
+
+/* Transfer-function phase arithmetic */
+enum {
+    MSOSM_PHASE_FP64 = 0, /* msosm_prepare.cpp:107-164 */
+    MSOSM_PHASE_FP32 = 1  /* legacy float-w form, msosm_prepare.cpp:166-213 */
+};
+
+const char *msosm_last_error(void);
+int msosm_version(void);
+
+/* ------------------------------------------------------------------------------------------------
+ * Design (host). Bit-identical to the reference's host arithmetic for Nd, M, delaycount,
+ * segment boundaries and segments; the transfer function is computed in FP64 and rounded once.
+ * --------------------------------------------------------------------------------------------- */
+
+/* Prepare_MSOSM::Prepare_MSOSM Nd (dedispersion/msosm/msosm_prepare.cpp:12,33) */
+int msosm_design_nd(float bw, float f0, float dm, int *nd);
+/* Prepare_MSOSM::calculate_min_order, per-DM M_i (msosm_prepare.cpp:48-69) */
+int msosm_design_min_order(int nd, int *M);
+/* Prepare_OSM::calculate_min_order (dedispersion/overlap_save/osm_prepare.cpp:17-30) */
+int msosm_design_osm_order(unsigned long nd, unsigned long *M);
+/* delaycount of one DM (msosm_prepare.cpp:217); Nd == 0 (DM 0) gives 1 */
+int msosm_design_delaycount(int nd, int M, int *delaycount);
+/* Prepare_MSOSM::segmentation(i) (msosm_prepare.cpp:215-284): delay_points[2M] in blocks.
+ * DM 0 yields all-zero delays (the reference leaves them uninitialised). */
+int msosm_design_segmentation(float bw, float f0, float dm, int M, int *delay_points, int *delaycount);
+/* MSOSM_GPU_BATCH::build_segments (dedispersion/msosm/msosm_gpu_batch.cpp:262-295):
+ * (start, len, group) triples into seg[3*cap]. */
+int msosm_design_build_segments(const int *delay_points, int fft_len, int max_len, int *seg, int cap, int *count);
+/* Prepare_MSOSM::generate_dedisp_params(i) (msosm_prepare.cpp:107-164): H[2M] complex (2*2M floats). */
+int msosm_design_transfer_function(float bw, float f0, float dm, int M, int phase_mode, float *H);
+/* Same for num_dms DMs into H[num_dms][2M] using up to nthreads host threads (0 = all cores). */
+int msosm_design_transfer_functions(float bw, float f0, const float *dms, int num_dms, int M, int phase_mode,
+                                    float *H, int nthreads);
+
+/* ------------------------------------------------------------------------------------------------
+ * Device plan: the MSOSM_GPU_BATCH device pipeline (msosm_gpu_batch.cpp:12-151,178-245).
+ * One push consumes batch*M new samples and produces batch*M output samples per DM.
+ * --------------------------------------------------------------------------------------------- */
+typedef struct msosm_plan msosm_plan;
+
+typedef struct {
+    int M;          /* half FFT length (fftpoint = 2M, a power of two in [2^12, 2^22]) */
+    int num_dms;    /* DM trials D */
+    int batch;      /* blocks of M samples per push (B) */
+    int delaycount; /* max over DMs of delaycount; spectra ring = roundup(delaycount + B - 1, B) slots */
+    int in_format;  /* MSOSM_IN_* */
+    int out_flags;  /* MSOSM_OUT_* */
+    int device;     /* HIP device ordinal, -1 = current device */
+    void *stream;   /* hipStream_t to run on; NULL = a stream owned by the plan */
+} msosm_plan_desc;
+
+typedef struct {
+    int L, FBS, workgroups;
+    long long blocks_pushed;
+    size_t device_bytes;
+} msosm_plan_info;
+
+/* initialize_uint16 allocation half (msosm_gpu_batch.cpp:12-151); ring and history start at zero */
+int msosm_plan_create(const msosm_plan_desc *desc, msosm_plan **plan);
+/* frees device memory; never resets the device (cf. msosm_gpu_batch.cpp:257-260) */
+int msosm_plan_destroy(msosm_plan *plan);
+/* upload one DM's design: delay_points[2M] (values in [0, delaycount)) and H[2M] complex
+ * (msosm_gpu_batch.cpp:54-93) */
+int msosm_plan_set_dm(msosm_plan *plan, int dm_index, const int *delay_points, const float *H);
+/* filter_block_uint16 (msosm_gpu_batch.cpp:178-236) for any input format: batch*M samples from host
+ * (copied) or device memory (read in place; must stay valid until the next synchronize) */
+int msosm_plan_push(msosm_plan *plan, const void *input, int input_on_device);
+/* get_output_pointer (dedispersion/include/msosm_gpu_batch.h:24): complex64 [D][B][M] device buffer */
+int msosm_plan_output_c64(msosm_plan *plan, void **dev_ptr);
+/* uint16-pair [D][B][M] device buffer (save mode) */
+int msosm_plan_output_u16(msosm_plan *plan, void **dev_ptr);
+/* get_output (msosm_gpu_batch.cpp:238-245): D2H copy of one output buffer, ordered after the last push */
+int msosm_plan_get_output(msosm_plan *plan, int which, void *host_dst);
+/* synchronize (msosm_gpu_batch.cpp:247-250) */
+int msosm_plan_synchronize(msosm_plan *plan);
+/* forget the stream history: zero ring + input history, block counter = 0 */
+int msosm_plan_reset(msosm_plan *plan);
+int msosm_plan_get_stream(msosm_plan *plan, void **stream);
+int msosm_plan_get_info(msosm_plan *plan, msosm_plan_info *info);
+/* HIP-event timing of the plan's kernels. kernel: 0 = forward FFT, 1 = dedispersion */
+int msosm_plan_profile(msosm_plan *plan, int enable);
+int msosm_plan_profile_read(msosm_plan *plan, int kernel, double *total_ms, long long *launches);
+
+/* ------------------------------------------------------------------------------------------------
+ * Fold: Fold_GPU (folding/fold_gpu.cpp:4-68) generalised to num_series series (one per DM).
+ * --------------------------------------------------------------------------------------------- */
+typedef struct msosm_fold msosm_fold;
+
+/* Fold_GPU::Fold_GPU (fold_gpu.cpp:4-25): period [s], fs [Hz], size = samples per call,
+ * nbins (0 = period*fs samples, as time_bin = -1) */
+int msosm_fold_create(double period, double fs, unsigned long size, unsigned long nbins, int num_series, int device,
+                      msosm_fold **fold);
+int msosm_fold_destroy(msosm_fold *fold);
+/* calculate_intensity + fold_data_phase (fold_gpu.cpp:27-54): series s at pol1_dev + s*series_stride
+ * complex elements; pol2_dev != NULL selects the two-polarisation amplitude sqrt(|a|^2+|b|^2),
+ * otherwise the power |a|^2. stream: hipStream_t or NULL (legacy default stream). */
+int msosm_fold_accumulate(msosm_fold *fold, const void *pol1_dev, const void *pol2_dev, long long series_stride,
+                          void *stream);
+/* get_folded_data (fold_gpu.cpp:56-60): profile[num_series][nbins] = sum/count (0 where empty);
+ * any output pointer may be NULL */
+int msosm_fold_read(msosm_fold *fold, float *profile, double *sums, unsigned long long *counts);
+int msosm_fold_reset(msosm_fold *fold);
+int msosm_fold_get_nbins(msosm_fold *fold, unsigned long *nbins);
+/* fused fold: every following push folds the power of every DM's output inside the dedispersion
+ * kernel (size must equal batch*M, num_series == num_dms, nbins <= 2048). NULL detaches. */
+int msosm_plan_attach_fold(msosm_plan *plan, msosm_fold *fold);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSOSM_AMD_H */

@@ -1,0 +1,97 @@
+"""GPU parity of the MS-OSM path against the FP64 oracle (small sizes the oracle finishes in seconds).
+
+Bar: dedispersed series within rel-L2 1e-5 (FP32 arithmetic vs FP64); uint16 codes within one
+code step; fold bin counts bit-exact; folded sums within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import msosm_amd as ms
+from oracle import oracle as orc
+from tests.util import gpu_series, make_input, oracle_design, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+# cfg-1-like band (16 MHz at 1392-1408 MHz); M from the reference rule for the largest DM
+BW, F0 = 16e6, 1392e6
+
+
+def _run_case(fmt, dms, batch, calls, out=ms.OUT_C64, M=None, seed=3):
+    design = ms.Design(BW, F0, dms, M=M)
+    n = batch * design.M * calls
+    n2 = 1 << int(np.ceil(np.log2(n)))
+    raw, x = make_input(fmt, n2, BW, F0, dm=float(max(dms)), period=0.0011, seed=seed)
+    raw, x = raw[:n], x[:n]
+    print(f"case fmt={fmt} dms={list(dms)} M={design.M} dc={design.delaycount} batch={batch} calls={calls}",
+          flush=True)
+    got = gpu_series(design, raw, fmt, batch, out)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    assert np.array_equal(delays, design.delays)
+    ref = orc.msosm(x, design.M, delays, H)
+    return design, got, ref
+
+
+@pytest.mark.parametrize("fmt", [ms.IN_C64, ms.IN_I8, ms.IN_U16])
+def test_series_c64_matches_oracle(fmt):
+    design, got, ref = _run_case(fmt, [0.0, 20.0, 50.0], batch=4, calls=3)
+    for d in range(got.shape[0]):
+        err = rel_l2(got[d], ref[d])
+        print(f"  DM {design.dms[d]}: rel-L2 {err:.3e}", flush=True)
+        assert err < TOL
+
+
+def test_dm0_is_identity_delay_of_one_block():
+    # DM 0: H == 1 and no delays, so out[t] = x[t] exactly (up to FFT rounding)
+    design, got, ref = _run_case(ms.IN_C64, [0.0], batch=2, calls=2, M=4096)
+    assert rel_l2(got[0], ref[0]) < TOL
+
+
+def test_uint16_output_codes():
+    design, got, ref = _run_case(ms.IN_U16, [10.0, 50.0], batch=4, calls=2, out=ms.OUT_U16)
+    # oracle codes from the FP64 series, rounded as the encoder rounds
+    enc = np.vectorize(orc.encode_u16_pair, otypes=[np.uint32])
+    ref_codes = enc(ref.real.astype(np.float32), ref.imag.astype(np.float32))
+    lo_d = np.abs((got & 0xFFFF).astype(np.int64) - (ref_codes & 0xFFFF).astype(np.int64))
+    hi_d = np.abs((got >> 16).astype(np.int64) - (ref_codes >> 16).astype(np.int64))
+    print(f"  max code diff {max(lo_d.max(), hi_d.max())}, frac differing {np.mean(lo_d > 0):.2e}", flush=True)
+    assert max(lo_d.max(), hi_d.max()) <= 1
+    assert np.mean(lo_d > 0) < 1e-3
+
+
+def test_fused_fold_matches_oracle():
+    # DM 0 is left out: its output equals the input up to rounding, so integer-zero input samples
+    # sit at the I == 0 skip threshold where FP32 and FP64 legitimately disagree
+    dms = [5.0, 25.0, 50.0]
+    batch, calls, nbins, period = 4, 3, 256, 0.0011
+    design = ms.Design(BW, F0, dms)
+    n = batch * design.M * calls
+    raw, x = make_input(ms.IN_I8, 1 << int(np.ceil(np.log2(n))), BW, F0, 50.0, period, seed=11)
+    raw, x = raw[:n], x[:n]
+    plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_NONE)
+    fold = ms.Fold(period, BW, batch * design.M, nbins, num_series=len(dms))
+    plan.attach_fold(fold)
+    per = batch * design.M
+    for c in range(calls):
+        plan.push(raw[c * per:(c + 1) * per])
+    plan.synchronize()
+    prof, sums, cnts = fold.read()
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    dt = 1.0 / BW
+    for d in range(len(dms)):
+        rs = np.zeros(nbins)
+        rc = np.zeros(nbins, dtype=np.uint64)
+        t0 = 0.0
+        for c in range(calls):
+            seg = np.abs(ref[d, c * per:(c + 1) * per]) ** 2
+            orc.fold(seg, nbins, t0, dt, period, rs, rc)
+            t0 += float(per) * dt  # t_start += size * dt (fold_gpu.cpp:53)
+        print(f"  DM {dms[d]}: counts equal {np.array_equal(cnts[d], rc)}, sum rel err "
+              f"{np.max(np.abs(sums[d] - rs) / rs):.2e}", flush=True)
+        assert np.array_equal(cnts[d], rc)
+        assert np.max(np.abs(sums[d] - rs) / rs) < TOL
+        np.testing.assert_allclose(prof[d], rs / rc, rtol=TOL)
+    plan.close()
+    fold.close()

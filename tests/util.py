@@ -1,0 +1,51 @@
+"""Shared helpers for the parity tests: seeded inputs, GPU runs through the C ABI, oracle runs."""
+from __future__ import annotations
+
+import numpy as np
+
+import msosm_amd as ms
+from oracle import oracle as orc
+
+
+def rel_l2(a, b) -> float:
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def make_input(fmt: int, n: int, fs: float, f0: float, dm: float, period: float, seed: int = 7):
+    """Seeded dispersed pulsar in the requested input format.
+
+    Returns (raw samples as pushed to the GPU, the same samples as complex128 for the oracle)."""
+    x = orc.pulsar(n, fs, f0, dm, period, duty=0.03, amp=0.5, sigma=1.0, seed=seed)
+    if fmt == ms.IN_C64:
+        raw = x.astype(np.complex64)
+        return raw, raw.astype(np.complex128)
+    if fmt == ms.IN_I8:
+        q = orc.quantize_int8(x)
+        return q.reshape(-1, 2), orc.int8_to_complex(q)
+    if fmt == ms.IN_U16:
+        s = 2000.0 / np.sqrt(np.mean(np.abs(x) ** 2) / 2)
+        re = np.clip(np.floor(x.real * s + 32768.0), 0, 65535).astype(np.uint32)
+        im = np.clip(np.floor(x.imag * s + 32768.0), 0, 65535).astype(np.uint32)
+        packed = (im << 16) | re
+        return packed, orc.unpack_u16(packed)
+    raise ValueError(fmt)
+
+
+def oracle_design(bw, f0, dms, M):
+    delays = np.stack([orc.segmentation(bw, f0, float(np.float32(dm)), M)[0] for dm in dms])
+    H = np.stack([orc.transfer_function(bw, f0, float(np.float32(dm)), M) for dm in dms])
+    return delays, H
+
+
+def gpu_series(design: ms.Design, raw: np.ndarray, fmt: int, batch: int, out=ms.OUT_C64):
+    """Push raw through a plan call by call; returns the concatenated [D][N] output."""
+    M = design.M
+    per = batch * M
+    assert raw.shape[0] % per == 0
+    plan = ms.Plan.from_design(design, batch, in_format=fmt, out_flags=out)
+    outs = []
+    for c in range(raw.shape[0] // per):
+        plan.push(raw[c * per:(c + 1) * per])
+        outs.append(plan.output_c64() if out & ms.OUT_C64 else plan.output_u16())
+    plan.close()
+    return np.concatenate(outs, axis=1)
