@@ -243,7 +243,9 @@ struct msosm_plan {
     bool own_stream = false;
     int M = 0, L = 0, log2L = 0, D = 0, B = 0, dc = 0, FBS = 0, in_fmt = 0, out_flags = 0, G = 0;
     size_t in_bytes = 0;
-    void *in_dev = nullptr;
+    void *in_dev[2] = {nullptr, nullptr}; // staging buffers for host inputs
+    hipEvent_t in_ev[2] = {nullptr, nullptr};
+    long long pushes = 0;
     float2 *hist = nullptr, *ring = nullptr, *H = nullptr, *scratch = nullptr, *out_c64 = nullptr, *tw = nullptr;
     uint16_t *delay = nullptr, *bins = nullptr;
     uint32_t *out_u16 = nullptr;
@@ -271,9 +273,12 @@ static void plan_free(msosm_plan *p)
     if (!p) return;
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    void *bufs[] = {p->in_dev, p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay, p->bins, p->out_u16};
+    void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
+                    p->bins, p->out_u16};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (hipEvent_t e : p->in_ev)
+        if (e) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k)
         for (auto &e : p->pending[k]) p->pool.push_back(e);
     for (auto &e : p->pool) {
@@ -376,7 +381,10 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
 
     e = dalloc(p, &p->tw, ntw * sizeof(float2));
     if (e == hipSuccess) e = hipMemcpy(p->tw, tw.data(), ntw * sizeof(float2), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = dalloc(p, &p->in_dev, B * M * p->in_bytes);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = dalloc(p, &p->in_dev[i], B * M * p->in_bytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->in_ev[i], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = dalloc(p, &p->hist, M * sizeof(float2));
     if (e == hipSuccess) e = dalloc(p, &p->ring, (size_t)p->FBS * L * sizeof(float2));
     if (e == hipSuccess) e = dalloc(p, &p->H, D * L * sizeof(float2));
@@ -451,9 +459,11 @@ static int collect_profile(msosm_plan *p)
     return MSOSM_OK;
 }
 
-extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int on_device)
+extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
 {
     if (!p || !input) return set_error(MSOSM_ERR_ARG, "plan_push: null argument");
+    if (mode != MSOSM_INPUT_HOST_ASYNC && mode != MSOSM_INPUT_DEVICE && mode != MSOSM_INPUT_HOST_SYNC)
+        return set_error(MSOSM_ERR_ARG, "plan_push: unknown input mode");
     for (int i = 0; i < p->D; ++i)
         if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
     if (!p->fold && !(p->out_flags & (MSOSM_OUT_C64 | MSOSM_OUT_U16)))
@@ -461,9 +471,15 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int on_device)
     HIPCHK(hipSetDevice(p->device));
     const size_t nsamp = (size_t)p->B * p->M;
     const void *src = input;
-    if (!on_device) {
-        HIPCHK(hipMemcpyAsync(p->in_dev, input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
-        src = p->in_dev;
+    const int slot = (int)(p->pushes & 1);
+    if (mode != MSOSM_INPUT_DEVICE) {
+        // the staging buffer was last read by the push two calls ago; wait for that to finish
+        HIPCHK(hipEventSynchronize(p->in_ev[slot]));
+        if (mode == MSOSM_INPUT_HOST_SYNC)
+            HIPCHK(hipMemcpy(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice));
+        else
+            HIPCHK(hipMemcpyAsync(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
+        src = p->in_dev[slot];
     }
     const float2 *tw = p->tw;
     FwdArgs fa{};
@@ -488,6 +504,7 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int on_device)
         p->pending[0].push_back(ev);
     }
     HIPCHK(launch_hist(p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M, p->stream));
+    if (mode != MSOSM_INPUT_DEVICE) HIPCHK(hipEventRecord(p->in_ev[slot], p->stream));
     InvArgs ia{};
     ia.ring = p->ring;
     ia.FBS = p->FBS;
@@ -519,6 +536,7 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int on_device)
         p->pending[1].push_back(ev);
     }
     p->g0 += p->B;
+    p->pushes += 1;
     if (p->fold) p->fold->t_start += static_cast<double>(nsamp) * p->fold->dt; // fold_gpu.cpp:53
     return MSOSM_OK;
 }

@@ -20,6 +20,7 @@ IN_U16, IN_I8, IN_C64 = 0, 1, 2
 OUT_NONE, OUT_C64, OUT_U16 = 0, 1, 2
 PHASE_FP64, PHASE_FP32 = 0, 1
 IN_BYTES = {IN_U16: 4, IN_I8: 2, IN_C64: 8}
+INPUT_HOST_ASYNC, INPUT_DEVICE, INPUT_HOST_SYNC = 0, 1, 2
 
 # every symbol declared in include/msosm_amd.h
 EXPORTS = [
@@ -275,12 +276,12 @@ class Plan:
     def push(self, x) -> None:
         """Push batch*M samples: a numpy array (host) or an int device address."""
         if isinstance(x, int):
-            _check(lib().msosm_plan_push(self.h, C.c_void_p(x), 1))
+            _check(lib().msosm_plan_push(self.h, C.c_void_p(x), INPUT_DEVICE))
             return
         a = np.ascontiguousarray(x)
-        assert a.nbytes == self.B * self.M * IN_BYTES[self.in_format], "push: wrong input size"
-        self._keep = a  # host buffer must outlive the asynchronous copy
-        _check(lib().msosm_plan_push(self.h, C.c_void_p(a.ctypes.data), 0))
+        if a.nbytes != self.B * self.M * IN_BYTES[self.in_format]:
+            raise ValueError(f"push: expected {self.B * self.M} samples, got {a.nbytes} bytes")
+        _check(lib().msosm_plan_push(self.h, C.c_void_p(a.ctypes.data), INPUT_HOST_SYNC))
 
     def output_c64(self) -> np.ndarray:
         out = np.empty((self.D, self.B * self.M), dtype=np.complex64)

@@ -112,9 +112,16 @@ int msosm_plan_destroy(msosm_plan *plan);
 /* upload one DM's design: delay_points[2M] (values in [0, delaycount)) and H[2M] complex
  * (msosm_gpu_batch.cpp:54-93) */
 int msosm_plan_set_dm(msosm_plan *plan, int dm_index, const int *delay_points, const float *H);
-/* filter_block_uint16 (msosm_gpu_batch.cpp:178-236) for any input format: batch*M samples from host
- * (copied) or device memory (read in place; must stay valid until the next synchronize) */
-int msosm_plan_push(msosm_plan *plan, const void *input, int input_on_device);
+/* Where a pushed input lives */
+enum {
+    MSOSM_INPUT_HOST_ASYNC = 0, /* host memory copied asynchronously (pin it; keep it unchanged until the
+                                   next synchronize), as the save-mode stream pipeline does */
+    MSOSM_INPUT_DEVICE = 1,     /* device memory read in place; keep it valid until the next synchronize */
+    MSOSM_INPUT_HOST_SYNC = 2   /* host memory copied before the call returns (fold-mode cudaMemcpy) */
+};
+/* filter_block_uint16 (msosm_gpu_batch.cpp:178-236) for any input format: batch*M new samples.
+ * Host inputs go through two staging buffers, so a copy overlaps the previous push's kernels. */
+int msosm_plan_push(msosm_plan *plan, const void *input, int input_mode);
 /* get_output_pointer (dedispersion/include/msosm_gpu_batch.h:24): complex64 [D][B][M] device buffer */
 int msosm_plan_output_c64(msosm_plan *plan, void **dev_ptr);
 /* uint16-pair [D][B][M] device buffer (save mode) */

@@ -1,0 +1,88 @@
+// Drives the reference-compatible C++ classes with the call sequences of the reference's harnesses
+// and writes the results for tests/test_gpu_reference_api.py to compare with the oracle.
+//
+//   save  <bw> <f0> <batch> <dms.txt> <in.u16> <out.u16>       check_msosm.cpp:79-80,168-174
+//   fold  <bw> <f0> <batch> <dm> <period> <nbins> <in.u16> <profile.txt>   msosm_psrdada.cpp:101-149
+//   osm   <bw> <f0> <batch> <dm> <in.u16> <out.c64>            check_osm.cpp:66-67,127-133
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "fold_gpu.h"
+#include "msosm_gpu_batch.h"
+#include "osm_gpu_batch.h"
+
+static std::vector<uint16_pair> read_u16(const std::string &path)
+{
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    const size_t n = (size_t)f.tellg() / sizeof(uint16_pair);
+    f.seekg(0);
+    std::vector<uint16_pair> v(n);
+    f.read(reinterpret_cast<char *>(v.data()), n * sizeof(uint16_pair));
+    return v;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    const float bw = std::stof(argv[2]), f0 = std::stof(argv[3]);
+    const int batch = std::stoi(argv[4]);
+    if (mode == "save") {
+        std::vector<float> dms;
+        std::ifstream df(argv[5]);
+        for (float v; df >> v;) dms.push_back(v);
+        auto input = read_u16(argv[6]);
+        MSOSM_GPU_BATCH msosm(bw, dms.data(), f0, (int)dms.size());
+        msosm.initialize_uint16(0, batch);
+        const size_t process_len = (size_t)batch * msosm.M;
+        const size_t count = input.size() / process_len;
+        std::vector<uint16_pair> output(dms.size() * count * process_len);
+        for (size_t k = 0; k < count; k++) {
+            msosm.filter_block_uint16(input.data() + k * process_len);
+            msosm.get_output(output.data() + k * process_len * dms.size());
+            msosm.synchronize(); // pageable host buffers: finish each call before reusing them
+        }
+        msosm.synchronize();
+        std::ofstream of(argv[7], std::ios::binary);
+        of.write(reinterpret_cast<const char *>(output.data()), output.size() * sizeof(uint16_pair));
+        std::cout << "M " << msosm.M << " delaycount " << msosm.delaycount << " calls " << count << std::endl;
+    } else if (mode == "fold") {
+        const float dm = std::stof(argv[5]), period = std::stof(argv[6]);
+        const unsigned long nbins = std::stoul(argv[7]);
+        auto input = read_u16(argv[8]);
+        MSOSM_GPU_BATCH msosm(bw, dm, f0);
+        msosm.initialize_uint16(0, batch, true);
+        const size_t process_len = (size_t)batch * msosm.M;
+        Fold_GPU fold(period, bw, process_len, argv[9], nbins);
+        for (size_t k = 0; k < input.size() / process_len; k++) {
+            msosm.filter_block_uint16(input.data() + k * process_len);
+            fold.calculate_intensity<MSOSM_GPU_BATCH>(&msosm);
+            fold.fold_data_phase();
+        }
+        fold.get_folded_data();
+        fold.write_to_file();
+        std::cout << "M " << msosm.M << " bins " << fold.time_bin << std::endl;
+    } else if (mode == "osm") {
+        const float dm = std::stof(argv[5]);
+        auto input = read_u16(argv[6]);
+        OSM_GPU_BATCH osm(bw, dm, f0);
+        osm.initialize_uint16(0, batch);
+        const size_t process_len = (size_t)batch * osm.M;
+        const size_t count = input.size() / process_len;
+        std::vector<Complex> output(count * process_len);
+        for (size_t k = 0; k < count; k++) {
+            osm.filter_block_uint16(input.data() + k * process_len);
+            osm.get_output(output.data() + k * process_len);
+        }
+        std::ofstream of(argv[7], std::ios::binary);
+        of.write(reinterpret_cast<const char *>(output.data()), output.size() * sizeof(Complex));
+        std::cout << "M " << osm.M << " calls " << count << std::endl;
+    } else {
+        return 2;
+    }
+    return 0;
+}

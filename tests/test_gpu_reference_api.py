@@ -1,0 +1,85 @@
+"""The reference-compatible C++ classes (MSOSM_GPU_BATCH, OSM_GPU_BATCH, Fold_GPU), driven by
+tests/cpp/check_reference_api.cpp with the reference harnesses' call sequences, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import msosm_amd as ms
+from oracle import oracle as orc
+from tests.util import make_input, oracle_design, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "cpp", "_build", "check_reference_api")
+BW, F0 = 16e6, 1392e6
+
+
+def _run(args, tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} not built (make -C tests/cpp)")
+    print("driver:", " ".join(map(str, args)), flush=True)
+    r = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    return r.stdout
+
+
+def test_save_mode_uint16(tmp_path):
+    dms = [12.5, 30.0, 50.0]
+    batch, calls = 4, 3
+    design = ms.Design(BW, F0, dms)
+    n = batch * design.M * calls
+    raw, x = make_input(ms.IN_U16, 1 << (n - 1).bit_length(), BW, F0, 50.0, 0.0011, seed=21)
+    raw, x = raw[:n], x[:n]
+    (tmp_path / "dms.txt").write_text("\n".join(str(d) for d in dms))
+    raw.astype(np.uint32).tofile(tmp_path / "in.u16")
+    _run(["save", BW, F0, batch, "dms.txt", "in.u16", "out.u16"], tmp_path)
+    got = np.fromfile(tmp_path / "out.u16", dtype=np.uint32)
+    per = batch * design.M
+    got = got.reshape(calls, len(dms), per).transpose(1, 0, 2).reshape(len(dms), n)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    enc = np.vectorize(orc.encode_u16_pair, otypes=[np.uint32])
+    ref_codes = enc(ref.real.astype(np.float32), ref.imag.astype(np.float32))
+    diff = np.maximum(np.abs((got & 0xFFFF).astype(np.int64) - (ref_codes & 0xFFFF)),
+                      np.abs((got >> 16).astype(np.int64) - (ref_codes >> 16)))
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+
+
+def test_fold_mode_profile(tmp_path):
+    dm, batch, calls, period, nbins = 40.0, 4, 3, 0.0011, 128
+    design = ms.Design(BW, F0, [dm])
+    n = batch * design.M * calls
+    raw, x = make_input(ms.IN_U16, 1 << (n - 1).bit_length(), BW, F0, dm, period, seed=22)
+    raw, x = raw[:n], x[:n]
+    raw.astype(np.uint32).tofile(tmp_path / "in.u16")
+    _run(["fold", BW, F0, batch, dm, period, nbins, "in.u16", "fold.txt"], tmp_path)
+    prof = np.loadtxt(tmp_path / "fold.txt")
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)[0]
+    per = batch * design.M
+    rs, rc = np.zeros(nbins), np.zeros(nbins, dtype=np.uint64)
+    t0, dt = 0.0, 1.0 / float(np.float32(BW))
+    for c in range(calls):
+        orc.fold(np.abs(ref[c * per:(c + 1) * per]) ** 2, nbins, t0, dt, float(np.float32(period)), rs, rc)
+        t0 += float(per) * dt
+    np.testing.assert_allclose(prof, rs / rc, rtol=1e-5)
+
+
+def test_plain_osm(tmp_path):
+    dm, batch, calls = 50.0, 1, 4
+    nd = orc.nd(BW, F0, dm)
+    M = orc.osm_min_order(nd)  # 2^17: FFT length 2^18
+    n = batch * M * calls
+    raw, x = make_input(ms.IN_U16, n, BW, F0, dm, 0.033, seed=23)
+    raw.astype(np.uint32).tofile(tmp_path / "in.u16")
+    _run(["osm", BW, F0, batch, dm, "in.u16", "out.c64"], tmp_path)
+    got = np.fromfile(tmp_path / "out.c64", dtype=np.complex64)
+    H = orc.transfer_function(BW, F0, dm, M)[None, :]
+    ref = orc.msosm(x, M, np.zeros((1, 2 * M), np.int32), H)[0]
+    err = rel_l2(got, ref)
+    print(f"OSM L=2^18 rel-L2 {err:.2e}")
+    assert err < 1e-5
