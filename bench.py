@@ -61,7 +61,20 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               f"path (DM sharding, no exchange) is not built.", file=sys.stderr)
@@ -83,6 +96,8 @@ def main():
     N, B, M, D = W["samples"], W["batch"], design.M, W["num_dms"]
     per = B * M
     pushes = N // per
+    if args.pushes:
+        pushes = min(pushes, args.pushes)
     torch.cuda.init()
     dev = torch.device("cuda:0")
     gen = torch.Generator(device=dev).manual_seed(1234)
@@ -116,7 +131,7 @@ def main():
     fwd_ms, fwd_n = plan.profile_read(0)
     plan.profile(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = D * N / ((t1 - t0) / args.steps)
+    value = D * pushes * per / ((t1 - t0) / args.steps)
 
     # roofline of the dominant kernel: algorithmic bytes per launch = D*B*M DM-samples x 16 B
     # (the L = 2M gathered complex64 bins read per M output samples; the fold writes ~nothing)

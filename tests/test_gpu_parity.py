@@ -95,3 +95,53 @@ def test_fused_fold_matches_oracle():
         np.testing.assert_allclose(prof[d], rs / rc, rtol=TOL)
     plan.close()
     fold.close()
+
+
+@pytest.mark.parametrize("log2L", list(range(12, 23)))
+def test_every_fft_length(log2L):
+    # every compiled (N1, N2) shape: one DM whose smear spans a few blocks at this M
+    M = 1 << (log2L - 1)
+    dm = 30.0 * M / 8192
+    design = ms.Design(BW, F0, [dm], M=M)
+    calls = 3
+    n = design.M * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, dm, 0.0011, seed=log2L)
+    raw, x = raw[:n], x[:n]
+    print(f"L=2^{log2L} dc={design.delaycount}", flush=True)
+    got = gpu_series(design, raw, ms.IN_C64, 1)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    err = rel_l2(got[0], ref[0])
+    print(f"  rel-L2 {err:.3e}", flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("batch", [1, 3])
+def test_ring_wraps_many_times(batch):
+    # FBS = roundup(dc + batch - 1, batch) slots; 24 blocks wrap the ring several times
+    design = ms.Design(BW, F0, [6.0, 10.0], M=2048)
+    calls = 24 // batch
+    n = batch * design.M * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 10.0, 0.0011, seed=batch)
+    raw, x = raw[:n], x[:n]
+    got = gpu_series(design, raw, ms.IN_C64, batch)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    for d in range(2):
+        assert rel_l2(got[d], ref[d]) < TOL
+
+
+def test_fp32_phase_variant_runs_and_is_less_accurate():
+    # same data through the FP64-phase and the legacy FP32-phase design: the FP32 phase is off
+    dms = [50.0]
+    d64 = ms.Design(BW, F0, dms)
+    d32 = ms.Design(BW, F0, dms, phase_mode=ms.PHASE_FP32)
+    n = 2 * d64.M * 2
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 50.0, 0.0011, seed=9)
+    raw, x = raw[:n], x[:n]
+    delays, H = oracle_design(BW, F0, d64.dms, d64.M)
+    ref = orc.msosm(x, d64.M, delays, H)
+    e64 = rel_l2(gpu_series(d64, raw, ms.IN_C64, 2)[0], ref[0])
+    e32 = rel_l2(gpu_series(d32, raw, ms.IN_C64, 2)[0], ref[0])
+    print(f"  FP64 phase rel-L2 {e64:.2e}, FP32 phase rel-L2 {e32:.2e}", flush=True)
+    assert e64 < TOL and e32 > 10 * e64
