@@ -138,20 +138,25 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2 *w)
     }
 }
 
+// One pad word after every 16: the radix-16 exchange writes with a stride of 16 elements and reads
+// with a stride of 1, and the pad turns the strided side into a stride of 17 (no LDS bank conflicts
+// for 64 banks x 4 B and 8-byte accesses; see the layout notes in kernels.hip).
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+
 // Index of element e of column col inside a tile's LDS region.
-// COLFAST: e * NCOL + col (lanes run along columns); otherwise col * (NS + PAD) + e.
+// COLFAST: lanes run along columns (pad(e) * NCOL + col); otherwise each column is a padded run.
 template <int NS, int NCOL, bool COLFAST>
 __device__ __forceinline__ int lds_index(int e, int col)
 {
     if constexpr (COLFAST)
-        return e * NCOL + col;
+        return lds_pad(e) * NCOL + col;
     else
-        return col * (NS + 1) + e; // one pad element per column keeps strided passes off one bank
+        return col * (NS + NS / 16) + lds_pad(e);
 }
 
 template <int NS, int NCOL, bool COLFAST>
 struct TileLDS {
-    static constexpr int ELEMS = COLFAST ? NS * NCOL : NCOL * (NS + 1);
+    static constexpr int ELEMS = (NS + NS / 16) * NCOL;
 };
 
 // Radix of the pass that starts at span Ns.
@@ -174,20 +179,16 @@ __device__ __forceinline__ void butterflies(float2 (&v)[16], int jt, const float
             const int k = j & (Ns - 1);
             float2 w1 = tw[k * (NS / (Ns * R))];
             if (DIR < 0) w1 = cconj(w1);
-            // w^t from the binary powers w, w^2, w^4, w^8 (at most 3 products per twiddle)
-            float2 p2 = cmul(w1, w1);
-            float2 p4 = cmul(p2, p2);
-            float2 p8 = cmul(p4, p4);
+            // each power w^t once (R-2 products, depth <= 4: w^t = w^{t-hb} * w^{hb}, hb = highest bit)
+            float2 w[R];
+            w[1] = w1;
 #pragma unroll
-            for (int t = 1; t < R; ++t) {
-                float2 w = make_float2(1.f, 0.f);
-                bool first = true;
-                if (t & 1) { w = w1; first = false; }
-                if (t & 2) { w = first ? p2 : cmul(w, p2); first = false; }
-                if (t & 4) { w = first ? p4 : cmul(w, p4); first = false; }
-                if (t & 8) { w = first ? p8 : cmul(w, p8); }
-                v[q * R + t] = cmul(v[q * R + t], w);
+            for (int t = 2; t < R; ++t) {
+                const int hb = (t & 8) ? 8 : (t & 4) ? 4 : (t & 2) ? 2 : 1;
+                w[t] = (t == hb) ? cmul(w[hb / 2], w[hb / 2]) : cmul(w[hb], w[t - hb]);
             }
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[q * R + t] = cmul(v[q * R + t], w[t]);
         }
         dft<DIR, R>(&v[q * R]);
     }

@@ -23,11 +23,13 @@ constexpr int WG = 256 * SUBT;
 
 template <int NS>
 struct TileWords {
-    // LDS float2 words one tile needs in either pass layout (+ the dense transpose buffer)
+    // LDS float2 words one tile needs in either pass layout and in the padded sample-order buffer
     static constexpr int NC = TILE / NS;
     static constexpr int A = TileLDS<NS, NC, true>::ELEMS;
     static constexpr int B = TileLDS<NS, NC, false>::ELEMS;
-    static constexpr int V = (A > B ? A : B) > TILE ? (A > B ? A : B) : TILE;
+    static constexpr int T = TILE + TILE / 16;
+    static constexpr int AB = A > B ? A : B;
+    static constexpr int V = AB > T ? AB : T;
 };
 
 template <int N1, int N2>
@@ -41,6 +43,20 @@ struct Shape {
     static constexpr int TILE_WORDS = WA > WB ? WA : WB;
     static constexpr int LDS_TILES_BYTES = SUBT * TILE_WORDS * 8;
 };
+
+// Element idx of an array whose byte size is below 4 GiB: a 32-bit byte offset from a (uniform)
+// base lets the compiler use the SGPR-base + 32-bit VGPR-offset addressing form, with no per-lane
+// 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, uint32_t idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T *base, uint32_t idx, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
 
 struct Tw {
     const float2 *n1;  // e^{2 pi i m / N1}, m < N1
@@ -87,22 +103,30 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
         }
         subfft<NS, NC, true, DIR>(v, lds, col, jt, tw.n2);
         if (active) {
-            constexpr int R = FinalShape<NS>::R;
+            // element (q, t) is output n2 = jt + q*TPC + t*NsL; its twiddle W^{n2 k1} is
+            // W^{(jt + q*TPC) k1} (table) times c^t with c = W^{NsL k1} from binary powers, so a
+            // thread does 2 + 2*NB table reads per tile instead of two per element
+            constexpr int R = FinalShape<NS>::R, NsL = NS / R;
+            const float2 c = big_twiddle<DIR>(tw, k1 * NsL);
 #pragma unroll
-            for (int q = 0; q < 16 / R; ++q)
+            for (int q = 0; q < 16 / R; ++q) {
+                float2 w = big_twiddle<DIR>(tw, (jt + q * TPC) * k1);
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     const int n2 = final_pos<NS>(jt, q, t);
-                    const float2 w = big_twiddle<DIR>(tw, n2 * k1);
-                    scr[((size_t)tile * N2 + n2) * NC + col] = cmul(v[q * R + t], w);
+                    st32(scr, (uint32_t)((tile * N2 + n2) * NC + col), cmul(v[q * R + t], w));
+                    if (t + 1 < R) w = cmul(w, c); // depth <= 15 products: < 1e-6 relative
                 }
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Pass B: row sub-FFTs of the scratch slot. For each tile, emit(lds, tile, active) is called with
-// the tile's outputs in LDS in sample order: lds[n1 * NCB + row] = y[n2 + N2 * n1], n2 = tile*NCB+row.
+// the tile's outputs in LDS in padded sample order: lds[lds_pad(f)] = y[n2 + N2 * n1] for
+// f = n1 * NCB + row, n2 = tile * NCB + row. Epilogues read f = lane + 256 * i, so consecutive
+// lanes hold consecutive samples (runs of NCB) and neither the LDS reads nor the stores conflict.
 // ---------------------------------------------------------------------------------------------
 template <int N1, int N2, int DIR, class Emit>
 __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emit &emit, float2 *lds_all, const Tw &tw)
@@ -124,7 +148,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const int k1 = jt + t * TPC;
-                v[t] = scr[((size_t)(k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)];
+                v[t] = ld32(scr, (uint32_t)(((k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)));
             }
         } else {
 #pragma unroll
@@ -137,7 +161,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
 #pragma unroll
             for (int q = 0; q < 16 / R; ++q)
 #pragma unroll
-                for (int t = 0; t < R; ++t) lds[final_pos<NS>(jt, q, t) * NC + row] = v[q * R + t];
+                for (int t = 0; t < R; ++t) lds[lds_pad(final_pos<NS>(jt, q, t) * NC + row)] = v[q * R + t];
         }
         __syncthreads();
         emit(lds, tile, active);
@@ -149,16 +173,17 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ float2 decode_in(const void *in, int fmt, long long s)
 {
+    // one push is far below 4 GiB, so the sample offset is a 32-bit byte offset
     if (fmt == MSOSM_IN_U16) {
         // uint16 pair, offset binary with code 0 reserved for 0 (gpu_kernel.cu:22-29)
-        const uint32_t u = reinterpret_cast<const uint32_t *>(in)[s];
+        const uint32_t u = ld32(reinterpret_cast<const uint32_t *>(in), (uint32_t)s);
         const uint32_t a = u & 0xFFFFu, b = u >> 16;
         return make_float2((float)a - (a ? 32768.f : 0.f), (float)b - (b ? 32768.f : 0.f));
     } else if (fmt == MSOSM_IN_I8) {
-        const char2 c = reinterpret_cast<const char2 *>(in)[s];
+        const char2 c = ld32(reinterpret_cast<const char2 *>(in), (uint32_t)s);
         return make_float2((float)c.x, (float)c.y);
     }
-    return reinterpret_cast<const float2 *>(in)[s];
+    return ld32(reinterpret_cast<const float2 *>(in), (uint32_t)s);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -190,9 +215,9 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
             const int lt = threadIdx.x & 255;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const int f = lt * 16 + i;
+                const int f = lt + 256 * i;
                 const int n1 = f / NC, row = f % NC;
-                dst[(size_t)tile * NC + row + (size_t)N2 * n1] = lds[f];
+                st32(dst, (uint32_t)(tile * NC + row + N2 * n1), lds[lds_pad(f)]);
             }
         };
         pass_b<N1, N2, -1>(scr, emit, lds_all, tw);
@@ -211,7 +236,7 @@ __device__ __forceinline__ uint32_t enc_u16(float x)
     return (uint32_t)rintf(v);
 }
 
-template <int N1, int N2>
+template <int N1, int N2, bool RING32>
 __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
 {
     using S = Shape<N1, N2>;
@@ -243,9 +268,10 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
         const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
         const float2 *__restrict__ ring = a.ring;
         auto load = [&](int k) -> float2 {
-            int src = gm - (int)dl[k]; // delays are < FBS, so one wrap at most
+            int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
             if (src < 0) src += FBS;
-            return cmul(ring[(size_t)src * L + k], Hd[k]);
+            const float2 x = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + (uint32_t)k) : ring[(size_t)src * L + k];
+            return cmul(x, ld32(Hd, (uint32_t)k));
         };
         pass_a<N1, N2, +1>(load, scr, lds_all, tw);
         __syncthreads();
@@ -256,25 +282,38 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
             if (!active) return;
             constexpr int NC = S::NCB;
             const int lt = threadIdx.x & 255;
-            // upper half of the tile's outputs (n1 >= N1/2), 8 consecutive f per thread
+            // upper half of the tile's outputs (n1 >= N1/2): 8 consecutive f per thread, i.e. runs
+            // of consecutive samples; the fold accumulates a run in registers and flushes on a bin change
             const int fbase = (N1 / 2) * NC + lt * 8;
             int cur = -1;
             double csum = 0.0;
             unsigned int ccnt = 0;
+            // runs of >= 8 consecutive samples: their 8 bins are one aligned 16-byte load
+            uint16_t bv[8];
+            if (fold && NC >= 8) {
+                const int n1 = fbase / NC, row = fbase % NC;
+                const uint4 b4 = *reinterpret_cast<const uint4 *>(bins + (tile * NC + row + N2 * n1 - M));
+                const uint32_t w4[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    bv[2 * i] = (uint16_t)(w4[i] & 0xFFFFu);
+                    bv[2 * i + 1] = (uint16_t)(w4[i] >> 16);
+                }
+            }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int f = fbase + i;
                 const int n1 = f / NC, row = f % NC;
                 const int m = tile * NC + row + N2 * n1 - M;
-                float2 y = lds[f];
+                float2 y = lds[lds_pad(f)];
                 y.x *= scale;
                 y.y *= scale;
-                if (oc) oc[m] = y;
-                if (ou) ou[m] = (enc_u16(y.y) << 16) | enc_u16(y.x);
+                if (oc) st32(oc, (uint32_t)m, y);
+                if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
                 if (fold) {
                     const float I = y.x * y.x + y.y * y.y;
                     if (I != 0.0f) {
-                        const int bin = bins[m];
+                        const int bin = NC >= 8 ? (int)bv[i] : (int)bins[m];
                         if (bin != cur) {
                             if (ccnt) {
                                 atomicAdd(&hsum[cur], csum);
@@ -421,7 +460,11 @@ static hipError_t launch_inv_t(const InvArgs &a, int grid, hipStream_t st)
 {
     using S = Shape<N1, N2>;
     const size_t lds = S::LDS_TILES_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0);
-    hipLaunchKernelGGL((inv_kernel<N1, N2>), dim3(grid), dim3(WG), lds, st, a);
+    const bool ring32 = (unsigned long long)a.FBS * S::L * sizeof(float2) < (1ull << 32);
+    if (ring32)
+        hipLaunchKernelGGL((inv_kernel<N1, N2, true>), dim3(grid), dim3(WG), lds, st, a);
+    else
+        hipLaunchKernelGGL((inv_kernel<N1, N2, false>), dim3(grid), dim3(WG), lds, st, a);
     return hipGetLastError();
 }
 
@@ -531,7 +574,10 @@ hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes)
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&fwd_kernel<a_, b_>),                        \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, fwd_bytes);                     \
         if (e != hipSuccess) return e;                                                                                 \
-        return hipFuncSetAttribute(reinterpret_cast<const void *>(&inv_kernel<a_, b_>),                                \
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&inv_kernel<a_, b_, true>),                             \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, inv_bytes);                                \
+        if (e != hipSuccess) return e;                                                                                 \
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(&inv_kernel<a_, b_, false>),                         \
                                    hipFuncAttributeMaxDynamicSharedMemorySize, inv_bytes);                             \
     }
         MSOSM_SHAPES(X)
