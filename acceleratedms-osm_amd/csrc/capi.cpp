@@ -251,6 +251,7 @@ struct msosm_plan {
     uint32_t *out_u16 = nullptr;
     int tw_n1 = 0, tw_n2 = 0, tw_lo = 0, tw_hi = 0, tw_s = 0;
     long long g0 = 0;
+    int max_fold_bins = 0;
     msosm_fold *fold = nullptr;
     size_t device_bytes = 0;
     bool profile = false;
@@ -336,12 +337,13 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: device query: ") + hipGetErrorString(e));
+    // kernels use tiles + twiddle tables; the fused fold's histogram (12 B per bin) takes the rest
     const int tiles_lds = tiles_lds_bytes(log2L);
-    const int inv_lds = tiles_lds + 2048 * 12;
-    if (inv_lds > max_lds)
-        return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(inv_lds) +
+    if (tiles_lds > max_lds)
+        return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(tiles_lds) +
                                                " B of LDS per workgroup, device allows " + std::to_string(max_lds));
-    e = configure_lds(log2L, tiles_lds, inv_lds);
+    p->max_fold_bins = std::min(2048, (max_lds - tiles_lds) / 12);
+    e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12);
     if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: LDS attribute: ") + hipGetErrorString(e));
     p->G = std::max(8, (cus / 8) * 8);
 
@@ -643,7 +645,9 @@ extern "C" int msosm_plan_attach_fold(msosm_plan *p, msosm_fold *f)
         return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold size " + std::to_string(f->size) +
                                             " != batch*M = " + std::to_string(nsamp));
     if (f->S != p->D) return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold needs one series per DM");
-    if (f->nbins > 2048) return set_error(MSOSM_ERR_UNSUPPORTED, "plan_attach_fold: fused fold supports nbins <= 2048");
+    if (f->nbins > (unsigned long)p->max_fold_bins)
+        return set_error(MSOSM_ERR_UNSUPPORTED, "plan_attach_fold: the fused fold at this FFT length holds at most " +
+                                                    std::to_string(p->max_fold_bins) + " bins in LDS");
     if (f->device != p->device) return set_error(MSOSM_ERR_ARG, "plan_attach_fold: fold and plan on different devices");
     if (!p->bins) {
         hipError_t e = dalloc(p, &p->bins, nsamp * sizeof(uint16_t));

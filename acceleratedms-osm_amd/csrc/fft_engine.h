@@ -3,8 +3,8 @@
 // A sub-FFT of NS points (32..4096) is spread over NS/16 threads holding 16 points each; NCOL such
 // sub-FFTs ("columns") form a 4096-point tile worked by 256 threads. Passes are radix-16 (one
 // butterfly per thread) followed by one radix-2/4/8/16 pass; between passes the points go through
-// LDS. Twiddles inside a pass come from one table lookup per butterfly and short power chains
-// (depth <= 4 complex multiplies), so every twiddle stays within a few ulp.
+// LDS. Twiddles inside a pass come from one table read per butterfly (FP64-generated, rounded
+// once) and its powers (at most 4 products deep).
 //
 // Sign convention: DIR = -1 computes sum_k x[k] e^{-2 pi i nk/N} (forward, as cuFFT/FFTW FORWARD),
 // DIR = +1 computes the e^{+2 pi i nk/N} sum (inverse, unnormalised).

@@ -42,7 +42,14 @@ struct Shape {
     static constexpr int WA = TileWords<N2>::V, WB = TileWords<N1>::V;
     static constexpr int TILE_WORDS = WA > WB ? WA : WB;
     static constexpr int LDS_TILES_BYTES = SUBT * TILE_WORDS * 8;
+    // sub-FFT twiddle tables for N1 and N2 follow the tiles when they are small (<= 8 KiB; the
+    // largest lengths read them from global memory); the fold histogram comes last
+    static constexpr bool TW_IN_LDS = false; // measured: global (L1-resident) tables are faster here
+    static constexpr int LDS_TW_BYTES = TW_IN_LDS ? (N1 + N2) * 8 : 0;
+    static constexpr int LDS_BYTES = LDS_TILES_BYTES + LDS_TW_BYTES;
 };
+
+
 
 // Element idx of an array whose byte size is below 4 GiB: a 32-bit byte offset from a (uniform)
 // base lets the compiler use the SGPR-base + 32-bit VGPR-offset addressing form, with no per-lane
@@ -65,6 +72,21 @@ struct Tw {
     const float2 *hi;  // e^{2 pi i m 2^s / L}, m < L / 2^s
     int s;
 };
+
+// copy the N1- and N2-point twiddle tables into LDS (read once per kernel)
+template <int N1, int N2>
+__device__ __forceinline__ void load_twiddles(float2 *lds_all, Tw &tw)
+{
+    using S = Shape<N1, N2>;
+    if constexpr (!S::TW_IN_LDS) return;
+    float2 *t1 = lds_all + S::LDS_TILES_BYTES / 8;
+    float2 *t2 = t1 + N1;
+    for (int i = threadIdx.x; i < N1; i += blockDim.x) t1[i] = tw.n1[i];
+    for (int i = threadIdx.x; i < N2; i += blockDim.x) t2[i] = tw.n2[i];
+    tw.n1 = t1;
+    tw.n2 = t2;
+    __syncthreads();
+}
 
 template <int DIR>
 __device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
@@ -91,7 +113,7 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
         const int col = lt % NC, jt = lt / NC;
         float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
-        const bool active = tile < S::NTA;
+        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA; // all sub-tiles busy when divisible
         const int k1 = tile * NC + col;
         float2 v[16];
         if (active) {
@@ -141,7 +163,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
         const int jt = lt % TPC, row = lt / TPC;
         float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
-        const bool active = tile < S::NTB;
+        const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
         const int n2 = tile * NC + row;
         float2 v[16];
         if (active) {
@@ -195,7 +217,8 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
     using S = Shape<N1, N2>;
     constexpr int L = S::L, M = L / 2;
     extern __shared__ float2 lds_all[];
-    const Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    load_twiddles<N1, N2>(lds_all, tw);
     float2 *scr = a.scratch + (size_t)blockIdx.x * L;
 #pragma unroll 1
     for (int b = blockIdx.x; b < a.B; b += gridDim.x) {
@@ -242,10 +265,11 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     using S = Shape<N1, N2>;
     constexpr int L = S::L, M = L / 2;
     extern __shared__ float2 lds_all[];
-    const Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    load_twiddles<N1, N2>(lds_all, tw);
     float2 *scr = a.scratch + (size_t)blockIdx.x * L;
     const bool fold = a.fold_sum != nullptr;
-    double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + S::LDS_TILES_BYTES);
+    double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + S::LDS_BYTES);
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
     if (fold) {
         for (int i = threadIdx.x; i < a.nbins; i += WG) {
@@ -451,7 +475,7 @@ template <int N1, int N2>
 static hipError_t launch_fwd_t(const FwdArgs &a, int grid, hipStream_t st)
 {
     using S = Shape<N1, N2>;
-    hipLaunchKernelGGL((fwd_kernel<N1, N2>), dim3(grid), dim3(WG), S::LDS_TILES_BYTES, st, a);
+    hipLaunchKernelGGL((fwd_kernel<N1, N2>), dim3(grid), dim3(WG), S::LDS_BYTES, st, a);
     return hipGetLastError();
 }
 
@@ -459,7 +483,7 @@ template <int N1, int N2>
 static hipError_t launch_inv_t(const InvArgs &a, int grid, hipStream_t st)
 {
     using S = Shape<N1, N2>;
-    const size_t lds = S::LDS_TILES_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0);
+    const size_t lds = S::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0);
     const bool ring32 = (unsigned long long)a.FBS * S::L * sizeof(float2) < (1ull << 32);
     if (ring32)
         hipLaunchKernelGGL((inv_kernel<N1, N2, true>), dim3(grid), dim3(WG), lds, st, a);
@@ -471,7 +495,7 @@ static hipError_t launch_inv_t(const InvArgs &a, int grid, hipStream_t st)
 template <int N1, int N2>
 static int lds_bytes_t()
 {
-    return Shape<N1, N2>::LDS_TILES_BYTES;
+    return Shape<N1, N2>::LDS_BYTES;
 }
 
 #ifndef MSOSM_SHAPES
