@@ -248,7 +248,8 @@ struct msosm_plan {
     long long pushes = 0;
     float2 *hist = nullptr, *ring = nullptr, *H = nullptr, *scratch = nullptr, *out_c64 = nullptr, *tw = nullptr;
     uint16_t *delay = nullptr, *bins = nullptr;
-    uint32_t *out_u16 = nullptr;
+    uint32_t *out_u16 = nullptr, *out_f16 = nullptr;
+    float *out_scale = nullptr;
     int tw_n1 = 0, tw_n2 = 0, tw_lo = 0, tw_hi = 0, tw_s = 0;
     long long g0 = 0;
     int max_fold_bins = 0;
@@ -259,6 +260,7 @@ struct msosm_plan {
     double prof_ms[2] = {0, 0};
     long long prof_n[2] = {0, 0};
     std::vector<char> dm_set;
+    int launch_mode = MSOSM_LAUNCH_PERSISTENT;
 };
 
 template <class T>
@@ -275,7 +277,7 @@ static void plan_free(msosm_plan *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
-                    p->bins, p->out_u16};
+                    p->bins, p->out_u16, p->out_f16, p->out_scale};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : p->in_ev)
@@ -304,7 +306,8 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         return set_error(MSOSM_ERR_ARG, "plan_create: delaycount must be in [1, 65535]");
     if (d->in_format < MSOSM_IN_U16 || d->in_format > MSOSM_IN_C64)
         return set_error(MSOSM_ERR_ARG, "plan_create: unknown input format");
-    if (d->out_flags & ~(MSOSM_OUT_C64 | MSOSM_OUT_U16)) return set_error(MSOSM_ERR_ARG, "plan_create: unknown output flags");
+    if (d->out_flags & ~(MSOSM_OUT_C64 | MSOSM_OUT_U16 | MSOSM_OUT_F16))
+        return set_error(MSOSM_ERR_ARG, "plan_create: unknown output flags");
 
     msosm_plan *p = new msosm_plan();
     int dev = d->device;
@@ -339,11 +342,11 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: device query: ") + hipGetErrorString(e));
     // kernels use tiles + twiddle tables; the fused fold's histogram (12 B per bin) takes the rest
     const int tiles_lds = tiles_lds_bytes(log2L);
-    if (tiles_lds > max_lds)
+    if (tiles_lds + INV_EXTRA_LDS > max_lds)
         return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(tiles_lds) +
                                                " B of LDS per workgroup, device allows " + std::to_string(max_lds));
-    p->max_fold_bins = std::min(2048, (max_lds - tiles_lds) / 12);
-    e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12);
+    p->max_fold_bins = std::min(2048, (max_lds - tiles_lds - INV_EXTRA_LDS) / 12);
+    e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12 + INV_EXTRA_LDS);
     if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: LDS attribute: ") + hipGetErrorString(e));
     p->G = std::max(8, (cus / 8) * 8);
 
@@ -394,6 +397,8 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (e == hipSuccess) e = dalloc(p, &p->scratch, (size_t)p->G * L * sizeof(float2));
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_U16)) e = dalloc(p, &p->out_u16, D * B * M * sizeof(uint32_t));
+    if (e == hipSuccess && (p->out_flags & MSOSM_OUT_F16)) e = dalloc(p, &p->out_f16, D * B * M * sizeof(uint32_t));
+    if (e == hipSuccess && (p->out_flags & MSOSM_OUT_F16)) e = dalloc(p, &p->out_scale, D * B * sizeof(float));
     if (e == hipSuccess) e = hipMemset(p->hist, 0, M * sizeof(float2));
     if (e == hipSuccess) e = hipMemset(p->ring, 0, (size_t)p->FBS * L * sizeof(float2));
     if (e == hipSuccess) e = hipMemset(p->H, 0, D * L * sizeof(float2));
@@ -468,7 +473,7 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
         return set_error(MSOSM_ERR_ARG, "plan_push: unknown input mode");
     for (int i = 0; i < p->D; ++i)
         if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
-    if (!p->fold && !(p->out_flags & (MSOSM_OUT_C64 | MSOSM_OUT_U16)))
+    if (!p->fold && !p->out_flags)
         return set_error(MSOSM_ERR_STATE, "plan_push: no output selected and no fold attached");
     HIPCHK(hipSetDevice(p->device));
     const size_t nsamp = (size_t)p->B * p->M;
@@ -518,6 +523,8 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
     ia.scratch = p->scratch;
     ia.out_c64 = p->out_c64;
     ia.out_u16 = p->out_u16;
+    ia.out_f16 = p->out_f16;
+    ia.out_scale = p->out_scale;
     ia.tw_n1 = fa.tw_n1;
     ia.tw_n2 = fa.tw_n2;
     ia.tw_lo = fa.tw_lo;
@@ -532,7 +539,26 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
         ia.nbins = (int)f->nbins;
     }
     if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
-    HIPCHK(launch_inv(p->log2L, ia, p->G, p->stream));
+    if (p->launch_mode == MSOSM_LAUNCH_PER_DM) {
+        // the plain form: one launch per DM over that DM's blocks
+        for (int d = 0; d < p->D; ++d) {
+            InvArgs one = ia;
+            one.D = 1;
+            one.H = ia.H + (size_t)d * p->L;
+            one.delay = ia.delay + (size_t)d * p->L;
+            if (ia.out_c64) one.out_c64 = ia.out_c64 + (size_t)d * p->B * p->M;
+            if (ia.out_u16) one.out_u16 = ia.out_u16 + (size_t)d * p->B * p->M;
+            if (ia.out_f16) one.out_f16 = ia.out_f16 + (size_t)d * p->B * p->M;
+            if (ia.out_scale) one.out_scale = ia.out_scale + (size_t)d * p->B;
+            if (ia.fold_sum) {
+                one.fold_sum = ia.fold_sum + (size_t)d * ia.nbins;
+                one.fold_cnt = ia.fold_cnt + (size_t)d * ia.nbins;
+            }
+            HIPCHK(launch_inv(p->log2L, one, std::min(p->B, p->G), p->stream));
+        }
+    } else {
+        HIPCHK(launch_inv(p->log2L, ia, p->G, p->stream));
+    }
     if (p->profile) {
         HIPCHK(hipEventRecord(ev.b, p->stream));
         p->pending[1].push_back(ev);
@@ -559,6 +585,15 @@ extern "C" int msosm_plan_output_u16(msosm_plan *p, void **ptr)
     return MSOSM_OK;
 }
 
+extern "C" int msosm_plan_output_f16(msosm_plan *p, void **data, void **scales)
+{
+    if (!p || !data || !scales) return set_error(MSOSM_ERR_ARG, "plan_output_f16: null argument");
+    if (!p->out_f16) return set_error(MSOSM_ERR_STATE, "plan_output_f16: plan has no FP16 output");
+    *data = p->out_f16;
+    *scales = p->out_scale;
+    return MSOSM_OK;
+}
+
 extern "C" int msosm_plan_get_output(msosm_plan *p, int which, void *host)
 {
     if (!p || !host) return set_error(MSOSM_ERR_ARG, "plan_get_output: null argument");
@@ -570,8 +605,13 @@ extern "C" int msosm_plan_get_output(msosm_plan *p, int which, void *host)
     } else if (which == MSOSM_OUT_U16) {
         if (!p->out_u16) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no uint16 output");
         HIPCHK(hipMemcpyAsync(host, p->out_u16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+    } else if (which == MSOSM_OUT_F16) {
+        if (!p->out_f16) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no FP16 output");
+        HIPCHK(hipMemcpyAsync(host, p->out_f16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipMemcpyAsync(static_cast<char *>(host) + n * sizeof(uint32_t), p->out_scale,
+                              (size_t)p->D * p->B * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     } else {
-        return set_error(MSOSM_ERR_ARG, "plan_get_output: which must be MSOSM_OUT_C64 or MSOSM_OUT_U16");
+        return set_error(MSOSM_ERR_ARG, "plan_get_output: which must be MSOSM_OUT_C64, MSOSM_OUT_U16 or MSOSM_OUT_F16");
     }
     return MSOSM_OK;
 }
@@ -654,5 +694,13 @@ extern "C" int msosm_plan_attach_fold(msosm_plan *p, msosm_fold *f)
         if (e != hipSuccess) return set_error(MSOSM_ERR_ALLOC, std::string("plan_attach_fold: ") + hipGetErrorString(e));
     }
     p->fold = f;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_set_launch_mode(msosm_plan *p, int mode)
+{
+    if (!p || (mode != MSOSM_LAUNCH_PERSISTENT && mode != MSOSM_LAUNCH_PER_DM))
+        return set_error(MSOSM_ERR_ARG, "plan_set_launch_mode: unknown mode");
+    p->launch_mode = mode;
     return MSOSM_OK;
 }

@@ -31,6 +31,8 @@ struct InvArgs {
     float2 *scratch;          // [grid][L]
     float2 *out_c64;          // [D][B][M] or null
     uint32_t *out_u16;        // [D][B][M] or null
+    uint32_t *out_f16;        // [D][B][M] FP16 pairs or null
+    float *out_scale;         // [D][B] scales of out_f16
     const uint16_t *bins;     // [B*M] fold bin of each call-relative sample, or null
     double *fold_sum;         // [D][nbins] or null (no fold)
     unsigned long long *fold_cnt;
@@ -42,6 +44,8 @@ struct InvArgs {
 void split_length(int log2L, int *n1, int *n2);
 bool length_supported(int log2L);
 int tiles_lds_bytes(int log2L);
+// LDS the inverse kernel needs beyond its tiles and fold histogram (FP16 scale reduction)
+constexpr int INV_EXTRA_LDS = 16;
 int workgroup_threads();
 hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes);
 hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st);

@@ -259,6 +259,15 @@ __device__ __forceinline__ uint32_t enc_u16(float x)
     return (uint32_t)rintf(v);
 }
 
+// FP16 pair of y / scale (exact power-of-two scaling, round-to-nearest-even, saturated at +-65504)
+__device__ __forceinline__ uint32_t enc_f16_pair(float2 y, float inv_scale)
+{
+    const float xr = fminf(fmaxf(y.x * inv_scale, -65504.f), 65504.f);
+    const float xi = fminf(fmaxf(y.y * inv_scale, -65504.f), 65504.f);
+    const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, hr) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+}
+
 template <int N1, int N2, bool RING32>
 __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
 {
@@ -269,8 +278,10 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     load_twiddles<N1, N2>(lds_all, tw);
     float2 *scr = a.scratch + (size_t)blockIdx.x * L;
     const bool fold = a.fold_sum != nullptr;
+    const bool f16 = a.out_f16 != nullptr;
     double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + S::LDS_BYTES);
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
+    float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1)); // FP16 scale reduction
     if (fold) {
         for (int i = threadIdx.x; i < a.nbins; i += WG) {
             hsum[i] = 0.0;
@@ -283,6 +294,7 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     const int v = (G % 8 == 0) ? ((w % 8) * (G / 8) + w / 8) : w;
     const long long U = (long long)a.D * a.B;
     const float scale = 1.0f / (float)L;
+    if (f16 && threadIdx.x == 0) *pw_lds = 0.f;
 #pragma unroll 1
     for (long long u = v; u < U; u += G) {
         const int d = (int)(u / a.B), b = (int)(u % a.B);
@@ -291,16 +303,35 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
         const float2 *__restrict__ Hd = a.H + (size_t)d * L;
         const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
         const float2 *__restrict__ ring = a.ring;
+        float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval)
         auto load = [&](int k) -> float2 {
             int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
             if (src < 0) src += FBS;
             const float2 x = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + (uint32_t)k) : ring[(size_t)src * L + k];
-            return cmul(x, ld32(Hd, (uint32_t)k));
+            const float2 y = cmul(x, ld32(Hd, (uint32_t)k));
+            if (f16) power += y.x * y.x + y.y * y.y;
+            return y;
         };
         pass_a<N1, N2, +1>(load, scr, lds_all, tw);
+        float inv_scale = 1.f, out_scale = 1.f;
+        if (f16) {
+            // block rms of the 1/L-scaled series: sqrt(sum |Y|^2) / L; scale = 2^floor(log2(rms))
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
+            if ((threadIdx.x & 63) == 0) atomicAdd(pw_lds, power);
+            __syncthreads();
+            const float tot = *pw_lds;
+            const float rms = sqrtf(tot) * scale;
+            int e2 = 0;
+            (void)frexpf(rms, &e2); // rms = m * 2^e2, m in [0.5, 1)
+            out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
+            inv_scale = 1.f / out_scale; // exact: a power of two
+            if (threadIdx.x == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
+        }
         __syncthreads();
         float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * M : nullptr;
         uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * M : nullptr;
+        uint32_t *oh = a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
         const uint16_t *bins = a.bins ? a.bins + (size_t)b * M : nullptr;
         auto emit = [&](const float2 *lds, int tile, bool active) {
             if (!active) return;
@@ -334,6 +365,7 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
                 y.y *= scale;
                 if (oc) st32(oc, (uint32_t)m, y);
                 if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
+                if (oh) st32(oh, (uint32_t)m, enc_f16_pair(y, inv_scale));
                 if (fold) {
                     const float I = y.x * y.x + y.y * y.y;
                     if (I != 0.0f) {
@@ -359,6 +391,7 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
         };
         pass_b<N1, N2, +1>(scr, emit, lds_all, tw);
         __syncthreads();
+        if (f16 && threadIdx.x == 0) *pw_lds = 0.f; // next unit's reduction (ordered by the barriers)
         if (fold) {
             double *gs = a.fold_sum + (size_t)d * a.nbins;
             unsigned long long *gc = a.fold_cnt + (size_t)d * a.nbins;
@@ -483,7 +516,7 @@ template <int N1, int N2>
 static hipError_t launch_inv_t(const InvArgs &a, int grid, hipStream_t st)
 {
     using S = Shape<N1, N2>;
-    const size_t lds = S::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0);
+    const size_t lds = S::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0) + INV_EXTRA_LDS;
     const bool ring32 = (unsigned long long)a.FBS * S::L * sizeof(float2) < (1ull << 32);
     if (ring32)
         hipLaunchKernelGGL((inv_kernel<N1, N2, true>), dim3(grid), dim3(WG), lds, st, a);

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmsosm_amd.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ALLOC = range(6)
 IN_U16, IN_I8, IN_C64 = 0, 1, 2
-OUT_NONE, OUT_C64, OUT_U16 = 0, 1, 2
+OUT_NONE, OUT_C64, OUT_U16, OUT_F16 = 0, 1, 2, 4
 PHASE_FP64, PHASE_FP32 = 0, 1
 IN_BYTES = {IN_U16: 4, IN_I8: 2, IN_C64: 8}
 INPUT_HOST_ASYNC, INPUT_DEVICE, INPUT_HOST_SYNC = 0, 1, 2
@@ -29,11 +29,16 @@ EXPORTS = [
     "msosm_design_segmentation", "msosm_design_build_segments", "msosm_design_transfer_function",
     "msosm_design_transfer_functions",
     "msosm_plan_create", "msosm_plan_destroy", "msosm_plan_set_dm", "msosm_plan_push", "msosm_plan_output_c64",
-    "msosm_plan_output_u16", "msosm_plan_get_output", "msosm_plan_synchronize", "msosm_plan_reset",
+    "msosm_plan_output_u16", "msosm_plan_output_f16", "msosm_plan_get_output", "msosm_plan_synchronize", "msosm_plan_reset",
     "msosm_plan_get_stream", "msosm_plan_get_info", "msosm_plan_profile", "msosm_plan_profile_read",
     "msosm_fold_create", "msosm_fold_destroy", "msosm_fold_accumulate", "msosm_fold_read", "msosm_fold_reset",
-    "msosm_fold_get_nbins", "msosm_plan_attach_fold",
+    "msosm_fold_get_nbins", "msosm_plan_attach_fold", "msosm_plan_set_launch_mode",
+    "msosm_design_uneven_order", "msosm_multi_create", "msosm_multi_destroy", "msosm_multi_get_info",
+    "msosm_multi_set_dm", "msosm_multi_set_launch_mode", "msosm_multi_push", "msosm_multi_output_c64",
+    "msosm_multi_get_output", "msosm_multi_fold", "msosm_multi_fold_read", "msosm_multi_synchronize",
+    "msosm_multi_profile", "msosm_multi_profile_read",
 ]
+LAUNCH_PERSISTENT, LAUNCH_PER_DM = 0, 1
 
 
 class MsosmError(RuntimeError):
@@ -45,6 +50,12 @@ class MsosmError(RuntimeError):
 class PlanDesc(C.Structure):
     _fields_ = [("M", C.c_int), ("num_dms", C.c_int), ("batch", C.c_int), ("delaycount", C.c_int),
                 ("in_format", C.c_int), ("out_flags", C.c_int), ("device", C.c_int), ("stream", C.c_void_p)]
+
+
+class MultiDesc(C.Structure):
+    _fields_ = [("num_dms", C.c_int), ("M", C.POINTER(C.c_int)), ("delaycount", C.POINTER(C.c_int)),
+                ("push_blocks", C.c_int), ("in_format", C.c_int), ("out_flags", C.c_int), ("device", C.c_int),
+                ("stream", C.c_void_p)]
 
 
 class PlanInfo(C.Structure):
@@ -87,6 +98,7 @@ def lib():
         "msosm_plan_push": ([vp, vp, C.c_int], C.c_int),
         "msosm_plan_output_c64": ([vp, C.POINTER(vp)], C.c_int),
         "msosm_plan_output_u16": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_plan_output_f16": ([vp, C.POINTER(vp), C.POINTER(vp)], C.c_int),
         "msosm_plan_get_output": ([vp, C.c_int, vp], C.c_int),
         "msosm_plan_synchronize": ([vp], C.c_int),
         "msosm_plan_reset": ([vp], C.c_int),
@@ -102,6 +114,21 @@ def lib():
         "msosm_fold_reset": ([vp], C.c_int),
         "msosm_fold_get_nbins": ([vp, C.POINTER(C.c_ulong)], C.c_int),
         "msosm_plan_attach_fold": ([vp, vp], C.c_int),
+        "msosm_plan_set_launch_mode": ([vp, C.c_int], C.c_int),
+        "msosm_design_uneven_order": ([C.c_int, ip], C.c_int),
+        "msosm_multi_create": ([C.POINTER(MultiDesc), C.POINTER(vp)], C.c_int),
+        "msosm_multi_destroy": ([vp], C.c_int),
+        "msosm_multi_get_info": ([vp, C.c_int, ip, ip, C.POINTER(C.c_longlong)], C.c_int),
+        "msosm_multi_set_dm": ([vp, C.c_int, ip, fp], C.c_int),
+        "msosm_multi_set_launch_mode": ([vp, C.c_int], C.c_int),
+        "msosm_multi_push": ([vp, vp, C.c_int], C.c_int),
+        "msosm_multi_output_c64": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
+        "msosm_multi_get_output": ([vp, C.c_int, vp], C.c_int),
+        "msosm_multi_fold": ([vp, C.c_double, C.c_double, C.c_ulong], C.c_int),
+        "msosm_multi_fold_read": ([vp, fp, dp, C.POINTER(C.c_ulonglong)], C.c_int),
+        "msosm_multi_synchronize": ([vp], C.c_int),
+        "msosm_multi_profile": ([vp, C.c_int], C.c_int),
+        "msosm_multi_profile_read": ([vp, C.c_int, dp, C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -180,6 +207,36 @@ def design_transfer_functions(bw: float, f0: float, dms, M: int, phase_mode: int
     _check(lib().msosm_design_transfer_functions(bw, f0, _fp(d), d.size, M, phase_mode, _fp(H.view(np.float32)),
                                                  nthreads))
     return H
+
+
+def design_uneven_order(nd: int) -> int:
+    v = C.c_int()
+    _check(lib().msosm_design_uneven_order(nd, C.byref(v)))
+    return v.value
+
+
+class UnevenDesign:
+    """Per-DM design: every DM at its own order (the reference rule per DM, FFT length 2^14..2^21)."""
+
+    def __init__(self, bw: float, f0: float, dms, phase_mode: int = PHASE_FP64):
+        self.bw, self.f0 = float(bw), float(f0)
+        self.dms = np.ascontiguousarray(dms, dtype=np.float32)
+        self.nd = np.array([design_nd(bw, f0, float(dm)) for dm in self.dms], dtype=np.int64)
+        self.M = np.array([design_uneven_order(int(n)) for n in self.nd], dtype=np.int64)
+        self.delays, self.H, self.delaycount = [], [], np.zeros(len(self.dms), dtype=np.int64)
+        for M in np.unique(self.M):
+            idx = np.nonzero(self.M == M)[0]
+            H = design_transfer_functions(bw, f0, self.dms[idx], int(M), phase_mode)
+            for j, i in enumerate(idx):
+                d, dc = design_segmentation(bw, f0, float(self.dms[i]), int(M))
+                self.delaycount[i] = dc
+                self._put(i, d, H[j])
+
+    def _put(self, i, d, h):
+        while len(self.delays) <= i:
+            self.delays.append(None)
+            self.H.append(None)
+        self.delays[i], self.H[i] = d, h
 
 
 class Design:
@@ -295,6 +352,17 @@ class Plan:
         self.synchronize()
         return out
 
+    def output_f16(self) -> tuple[np.ndarray, np.ndarray]:
+        """FP16 pairs [D][B*M] as complex64 values (half * scale), and the [D][B] scales."""
+        n = self.D * self.B * self.M
+        raw = np.empty(n + self.D * self.B, dtype=np.uint32)
+        _check(lib().msosm_plan_get_output(self.h, OUT_F16, C.c_void_p(raw.ctypes.data)))
+        self.synchronize()
+        halves = raw[:n].view(np.float16).astype(np.float32).reshape(self.D, self.B, self.M, 2)
+        scales = raw[n:].view(np.float32).reshape(self.D, self.B)
+        vals = (halves[..., 0] + 1j * halves[..., 1]) * scales[:, :, None]
+        return vals.reshape(self.D, self.B * self.M).astype(np.complex64), scales
+
     def device_output_c64(self) -> int:
         p = C.c_void_p()
         _check(lib().msosm_plan_output_c64(self.h, C.byref(p)))
@@ -323,6 +391,9 @@ class Plan:
     def profile(self, enable: bool = True):
         _check(lib().msosm_plan_profile(self.h, int(enable)))
 
+    def set_launch_mode(self, mode: int):
+        _check(lib().msosm_plan_set_launch_mode(self.h, mode))
+
     def profile_read(self, kernel: int):
         ms, n = C.c_double(), C.c_longlong()
         _check(lib().msosm_plan_profile_read(self.h, kernel, C.byref(ms), C.byref(n)))
@@ -331,6 +402,80 @@ class Plan:
     def close(self):
         if self.h:
             lib().msosm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Multi:
+    """Uneven DM grid: one plan per length class on a shared stream (msosm_multi_* in the C ABI)."""
+
+    def __init__(self, design: UnevenDesign, push_blocks: int, in_format: int = IN_C64, out_flags: int = OUT_C64,
+                 device: int = -1):
+        D = len(design.dms)
+        self._M = np.ascontiguousarray(design.M, dtype=np.int32)
+        self._dc = np.ascontiguousarray(design.delaycount, dtype=np.int32)
+        desc = MultiDesc(D, _ip(self._M), _ip(self._dc), push_blocks, in_format, out_flags, device, None)
+        h = C.c_void_p()
+        _check(lib().msosm_multi_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        self.D, self.in_format = D, in_format
+        nc, ps = C.c_int(), C.c_longlong()
+        _check(lib().msosm_multi_get_info(h, 0, None, C.byref(nc), C.byref(ps)))
+        self.num_classes, self.push_samples = nc.value, ps.value
+        for i in range(D):
+            d = np.ascontiguousarray(design.delays[i], dtype=np.int32)
+            hh = np.ascontiguousarray(design.H[i], dtype=np.complex64)
+            _check(lib().msosm_multi_set_dm(h, i, _ip(d), _fp(hh.view(np.float32))))
+        self.nbins = 0
+
+    def set_launch_mode(self, mode: int):
+        _check(lib().msosm_multi_set_launch_mode(self.h, mode))
+
+    def push(self, x) -> None:
+        if isinstance(x, int):
+            _check(lib().msosm_multi_push(self.h, C.c_void_p(x), INPUT_DEVICE))
+            return
+        a = np.ascontiguousarray(x)
+        if a.nbytes != self.push_samples * IN_BYTES[self.in_format]:
+            raise ValueError("push: wrong input size")
+        _check(lib().msosm_multi_push(self.h, C.c_void_p(a.ctypes.data), INPUT_HOST_SYNC))
+
+    def output(self, dm: int) -> np.ndarray:
+        out = np.empty(self.push_samples, dtype=np.complex64)
+        _check(lib().msosm_multi_get_output(self.h, dm, C.c_void_p(out.ctypes.data)))
+        return out
+
+    def fold(self, period: float, fs: float, nbins: int):
+        _check(lib().msosm_multi_fold(self.h, period, fs, nbins))
+        self.nbins = nbins
+
+    def fold_read(self):
+        n = self.D * self.nbins
+        prof, sums, cnts = np.empty(n, np.float32), np.empty(n, np.float64), np.empty(n, np.uint64)
+        _check(lib().msosm_multi_fold_read(self.h, _fp(prof), sums.ctypes.data_as(C.POINTER(C.c_double)),
+                                           cnts.ctypes.data_as(C.POINTER(C.c_ulonglong))))
+        sh = (self.D, self.nbins)
+        return prof.reshape(sh), sums.reshape(sh), cnts.reshape(sh)
+
+    def synchronize(self):
+        _check(lib().msosm_multi_synchronize(self.h))
+
+    def profile(self, enable: bool = True):
+        _check(lib().msosm_multi_profile(self.h, int(enable)))
+
+    def profile_read(self, kernel: int):
+        ms, n = C.c_double(), C.c_longlong()
+        _check(lib().msosm_multi_profile_read(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().msosm_multi_destroy(self.h)
             self.h = None
 
     def __del__(self):

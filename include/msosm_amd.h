@@ -45,7 +45,10 @@ enum {
 enum {
     MSOSM_OUT_NONE = 0,
     MSOSM_OUT_C64 = 1, /* complex64 [D][B][M] (fold mode buffer, msosm_gpu_batch.cpp:141-145) */
-    MSOSM_OUT_U16 = 2  /* uint16-pair [D][B][M] (save mode buffer, msosm_gpu_batch.cpp:146-150) */
+    MSOSM_OUT_U16 = 2, /* uint16-pair [D][B][M] (save mode buffer, msosm_gpu_batch.cpp:146-150) */
+    MSOSM_OUT_F16 = 4  /* FP16 pair [D][B][M] (re low half, im high half) + one float scale per (DM, block):
+                          sample = half * scale; the scale is the power of two at or below the block's rms
+                          (no reference counterpart; 4 B per sample instead of 8) */
 };
 
 /* Transfer-function phase arithmetic */
@@ -126,7 +129,10 @@ int msosm_plan_push(msosm_plan *plan, const void *input, int input_mode);
 int msosm_plan_output_c64(msosm_plan *plan, void **dev_ptr);
 /* uint16-pair [D][B][M] device buffer (save mode) */
 int msosm_plan_output_u16(msosm_plan *plan, void **dev_ptr);
-/* get_output (msosm_gpu_batch.cpp:238-245): D2H copy of one output buffer, ordered after the last push */
+/* FP16-pair [D][B][M] device buffer and its [D][B] float scales */
+int msosm_plan_output_f16(msosm_plan *plan, void **dev_data, void **dev_scales);
+/* get_output (msosm_gpu_batch.cpp:238-245): D2H copy of one output buffer, ordered after the last push.
+ * For MSOSM_OUT_F16, host_dst receives the [D][B][M] pairs followed by the [D][B] float scales. */
 int msosm_plan_get_output(msosm_plan *plan, int which, void *host_dst);
 /* synchronize (msosm_gpu_batch.cpp:247-250) */
 int msosm_plan_synchronize(msosm_plan *plan);
@@ -134,6 +140,13 @@ int msosm_plan_synchronize(msosm_plan *plan);
 int msosm_plan_reset(msosm_plan *plan);
 int msosm_plan_get_stream(msosm_plan *plan, void **stream);
 int msosm_plan_get_info(msosm_plan *plan, msosm_plan_info *info);
+/* How push launches the dedispersion kernel */
+enum {
+    MSOSM_LAUNCH_PERSISTENT = 0, /* one persistent launch over every (DM, block) unit (default) */
+    MSOSM_LAUNCH_PER_DM = 1      /* one launch per DM, as the reference's per-DM inverse loop
+                                    (msosm_gpu_batch.cpp:196-199); kept as the comparison form */
+};
+int msosm_plan_set_launch_mode(msosm_plan *plan, int mode);
 /* HIP-event timing of the plan's kernels. kernel: 0 = forward FFT, 1 = dedispersion */
 int msosm_plan_profile(msosm_plan *plan, int enable);
 int msosm_plan_profile_read(msosm_plan *plan, int kernel, double *total_ms, long long *launches);
@@ -161,6 +174,45 @@ int msosm_fold_get_nbins(msosm_fold *fold, unsigned long *nbins);
 /* fused fold: every following push folds the power of every DM's output inside the dedispersion
  * kernel (size must equal batch*M, num_series == num_dms, nbins <= 2048). NULL detaches. */
 int msosm_plan_attach_fold(msosm_plan *plan, msosm_fold *fold);
+
+/* ------------------------------------------------------------------------------------------------
+ * Uneven DM grids (no reference counterpart: the reference uses one M for every DM,
+ * msosm_prepare.cpp:67). Each DM keeps its own M; DMs of equal M run as one plan ("length class");
+ * all classes share one stream and one input stream, so a push dedisperses the same
+ * push_blocks * max(M) samples for every DM.
+ * --------------------------------------------------------------------------------------------- */
+typedef struct msosm_multi msosm_multi;
+
+typedef struct {
+    int num_dms;
+    const int *M;          /* per-DM half FFT length (power of two, FFT length 2^12..2^22) */
+    const int *delaycount; /* per-DM delaycount */
+    int push_blocks;       /* blocks of the largest M per push */
+    int in_format;         /* MSOSM_IN_* */
+    int out_flags;         /* MSOSM_OUT_* (MSOSM_OUT_C64 for per-DM series) */
+    int device;            /* -1 = current */
+    void *stream;          /* NULL = owned stream */
+} msosm_multi_desc;
+
+/* per-DM order: the reference's minimum order for this DM alone (msosm_prepare.cpp:59-61),
+ * clamped to FFT lengths 2^14..2^21 */
+int msosm_design_uneven_order(int nd, int *M);
+int msosm_multi_create(const msosm_multi_desc *desc, msosm_multi **multi);
+int msosm_multi_destroy(msosm_multi *multi);
+int msosm_multi_get_info(msosm_multi *multi, int dm, int *M, int *num_classes, long long *push_samples);
+/* delay_points and H hold 2*M[dm] entries */
+int msosm_multi_set_dm(msosm_multi *multi, int dm, const int *delay_points, const float *H);
+int msosm_multi_set_launch_mode(msosm_multi *multi, int mode);
+int msosm_multi_push(msosm_multi *multi, const void *input, int input_mode);
+/* complex64 series of one DM for the last push (push_samples values) */
+int msosm_multi_output_c64(msosm_multi *multi, int dm, void **dev_ptr);
+int msosm_multi_get_output(msosm_multi *multi, int dm, void *host_dst);
+/* fused fold of every DM into nbins phase bins (replaces any earlier fold) */
+int msosm_multi_fold(msosm_multi *multi, double period, double fs, unsigned long nbins);
+int msosm_multi_fold_read(msosm_multi *multi, float *profile, double *sums, unsigned long long *counts);
+int msosm_multi_synchronize(msosm_multi *multi);
+int msosm_multi_profile(msosm_multi *multi, int enable);
+int msosm_multi_profile_read(msosm_multi *multi, int kernel, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }

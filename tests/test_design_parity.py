@@ -97,3 +97,17 @@ def test_c_abi_argument_errors():
     with pytest.raises(ms.MsosmError) as e:
         ms.Plan(M=4096, num_dms=1, batch=1, delaycount=1, in_format=7)
     assert e.value.code == ms.ERR_ARG
+
+
+def test_uneven_order_rule():
+    # per DM: the reference's minimum order for that DM alone, clamped to FFT lengths 2^14..2^21
+    for bw, f0, dm in [(32e6, 784e6, 0.0), (32e6, 784e6, 1.0), (32e6, 784e6, 100.0), (32e6, 784e6, 1000.0),
+                       (50e6, 325e6, 1500.0)]:
+        nd = orc.nd(bw, f0, dm)
+        expect = min(1 << 20, max(1 << 13, orc.min_order(nd) if nd > 0 else 0))
+        assert ms.design_uneven_order(nd) == expect
+    d = ms.UnevenDesign(32e6, 784e6, np.linspace(0, 1000, 16))
+    assert d.M.min() == 1 << 13 and d.M.max() == 1 << 17
+    for i in range(16):
+        ref, dc = orc.segmentation(32e6, 784e6, float(d.dms[i]), int(d.M[i]))
+        assert np.array_equal(d.delays[i], ref) and d.delaycount[i] == dc

@@ -145,3 +145,51 @@ def test_fp32_phase_variant_runs_and_is_less_accurate():
     e32 = rel_l2(gpu_series(d32, raw, ms.IN_C64, 2)[0], ref[0])
     print(f"  FP64 phase rel-L2 {e64:.2e}, FP32 phase rel-L2 {e32:.2e}", flush=True)
     assert e64 < TOL and e32 > 10 * e64
+
+
+def test_fp16_pair_output_with_block_scales():
+    """FP16-pair output: each (DM, block) carries a power-of-two scale at or below the rms of the
+    block's length-2M inverse transform; the pairs are y / scale rounded to nearest even."""
+    dms = [0.0, 20.0, 50.0]
+    batch, calls = 4, 3
+    design = ms.Design(BW, F0, dms)
+    per = batch * design.M
+    n = per * calls
+    raw, x = make_input(ms.IN_I8, 1 << int(np.ceil(np.log2(n))), BW, F0, 50.0, 0.0011, seed=5)
+    raw, x = raw[:n], x[:n]
+    got, got_c64, scl = [], [], []
+    for mode in (ms.LAUNCH_PERSISTENT, ms.LAUNCH_PER_DM):
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_C64 | ms.OUT_F16)
+        plan.set_launch_mode(mode)
+        v, c, s = [], [], []
+        for k in range(calls):
+            plan.push(raw[k * per:(k + 1) * per])
+            vals, scales = plan.output_f16()
+            v.append(vals)
+            c.append(plan.output_c64())
+            s.append(scales)
+        plan.close()
+        got.append(np.concatenate(v, axis=1))
+        got_c64.append(np.concatenate(c, axis=1))
+        scl.append(np.concatenate(s, axis=1))
+    # both launch forms write identical pairs and scales
+    assert np.array_equal(got[0], got[1]) and np.array_equal(scl[0], scl[1])
+    vals, c64, scales = got[0], got_c64[0], scl[0]
+    mant, _ = np.frexp(scales)
+    assert np.all(mant == 0.5), "scales are powers of two"
+    # the codec is exact relative to the same kernel's complex64 series
+    s_full = np.repeat(scales, design.M, axis=1)
+    enc = lambda r: np.clip(r / s_full, -65504, 65504).astype(np.float16).astype(np.float32) * s_full
+    assert np.array_equal(vals.real, enc(c64.real)) and np.array_equal(vals.imag, enc(c64.imag))
+    # the scale follows the block's power
+    rms = np.sqrt(np.mean(np.abs(c64.reshape(len(dms), -1, design.M)) ** 2, axis=2))
+    ratio = rms / scales
+    print(f"  rms/scale in [{ratio.min():.2f}, {ratio.max():.2f}]", flush=True)
+    assert np.all(ratio > 0.25) and np.all(ratio < 4.0)
+    # against the FP64 oracle: FP16 keeps 11 significant bits, unit roundoff 2^-11 per component
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    for d in range(len(dms)):
+        err = rel_l2(vals[d], ref[d])
+        print(f"  DM {dms[d]}: FP16 rel-L2 {err:.3e}", flush=True)
+        assert err < 2.0 ** -11

@@ -1,0 +1,90 @@
+"""Uneven DM grids (each DM at its own MS-OSM length) and the one-launch-per-DM form, against the
+FP64 oracle evaluated per DM at that DM's length."""
+import numpy as np
+import pytest
+
+import msosm_amd as ms
+from oracle import oracle as orc
+from tests.util import make_input, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BW, F0 = 16e6, 1392e6
+TOL = 1e-5
+
+
+def _oracle(design, x, i):
+    M = int(design.M[i])
+    dm = float(design.dms[i])
+    d, _ = orc.segmentation(BW, F0, dm, M)
+    H = orc.transfer_function(BW, F0, dm, M)
+    return orc.msosm(x, M, d[None], H[None])[0]
+
+
+def test_uneven_lengths_match_oracle_per_dm():
+    design = ms.UnevenDesign(BW, F0, [10.0, 200.0, 800.0, 0.0, 450.0])
+    assert len(set(design.M.tolist())) >= 3  # several length classes
+    multi = ms.Multi(design, push_blocks=2)
+    calls = 3
+    n = multi.push_samples * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 450.0, 0.0011, seed=31)
+    raw, x = raw[:n], x[:n]
+    outs = [[] for _ in range(multi.D)]
+    for c in range(calls):
+        multi.push(raw[c * multi.push_samples:(c + 1) * multi.push_samples])
+        for i in range(multi.D):
+            outs[i].append(multi.output(i))
+    print(f"classes {multi.num_classes}, M per DM {design.M.tolist()}", flush=True)
+    for i in range(multi.D):
+        got = np.concatenate(outs[i])
+        err = rel_l2(got, _oracle(design, x, i))
+        print(f"  DM {design.dms[i]} (M={design.M[i]}): rel-L2 {err:.2e}", flush=True)
+        assert err < TOL
+    multi.close()
+
+
+def test_uneven_fold_counts_match_oracle():
+    design = ms.UnevenDesign(BW, F0, [30.0, 300.0, 700.0])
+    multi = ms.Multi(design, push_blocks=2, out_flags=ms.OUT_NONE)
+    nbins, period = 128, 0.0011
+    multi.fold(period, BW, nbins)
+    calls = 2
+    n = multi.push_samples * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 300.0, period, seed=32)
+    raw, x = raw[:n], x[:n]
+    for c in range(calls):
+        multi.push(raw[c * multi.push_samples:(c + 1) * multi.push_samples])
+    multi.synchronize()
+    prof, sums, cnts = multi.fold_read()
+    dt = 1.0 / BW
+    for i in range(multi.D):
+        ref = _oracle(design, x, i)
+        rs, rc = np.zeros(nbins), np.zeros(nbins, dtype=np.uint64)
+        t0 = 0.0
+        for c in range(calls):
+            seg = ref[c * multi.push_samples:(c + 1) * multi.push_samples]
+            orc.fold(np.abs(seg) ** 2, nbins, t0, dt, period, rs, rc)
+            t0 += float(multi.push_samples) * dt
+        assert np.array_equal(cnts[i], rc)
+        assert np.max(np.abs(sums[i] - rs) / rs) < TOL
+    multi.close()
+
+
+def test_per_dm_launch_form_is_bitwise_identical():
+    # the comparison form (one launch per DM) runs the same arithmetic: outputs equal bit for bit
+    design = ms.Design(BW, F0, [5.0, 25.0, 50.0])
+    batch = 4
+    n = batch * design.M * 2
+    raw, _ = make_input(ms.IN_I8, 1 << (n - 1).bit_length(), BW, F0, 50.0, 0.0011, seed=33)
+    raw = raw[:n]
+    res = []
+    for mode in (ms.LAUNCH_PERSISTENT, ms.LAUNCH_PER_DM):
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_C64)
+        plan.set_launch_mode(mode)
+        o = []
+        for c in range(2):
+            plan.push(raw[c * batch * design.M:(c + 1) * batch * design.M])
+            o.append(plan.output_c64())
+        res.append(np.concatenate(o, axis=1))
+        plan.close()
+    assert np.array_equal(res[0], res[1])
