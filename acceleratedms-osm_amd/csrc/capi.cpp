@@ -12,6 +12,7 @@
 #include "capi_internal.h"
 #include "kernels.h"
 #include "msosm_amd.h"
+#include "plan_internal.h"
 
 namespace msosm {
 
@@ -118,6 +119,14 @@ extern "C" int msosm_design_transfer_functions(float bw, float f0, const float *
         (phase_mode != MSOSM_PHASE_FP64 && phase_mode != MSOSM_PHASE_FP32))
         return set_error(MSOSM_ERR_ARG, "design_transfer_functions: bad arguments");
     return design_transfer_functions(bw, f0, dms, num_dms, M, phase_mode, H, nthreads);
+}
+
+extern "C" int msosm_design_transfer_function_ddm(float bw, float f0, float dm, int M, int phase_mode, float *dH)
+{
+    if (!dH || !is_pow2(M) || M < 2 || (phase_mode != MSOSM_PHASE_FP64 && phase_mode != MSOSM_PHASE_FP32))
+        return set_error(MSOSM_ERR_ARG,
+                         "design_transfer_function_ddm: need M a power of two >= 2 and a valid phase mode");
+    return design_transfer_function(bw, f0, dm, M, phase_mode, dH, nullptr, 1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -261,6 +270,9 @@ struct msosm_plan {
     long long prof_n[2] = {0, 0};
     std::vector<char> dm_set;
     int launch_mode = MSOSM_LAUNCH_PERSISTENT;
+    // DM-derivative pass: dH/dDM per DM and the per-push primal series it is paired with
+    float2 *Hdot = nullptr, *tan_y = nullptr;
+    std::vector<char> tan_set;
 };
 
 template <class T>
@@ -277,7 +289,7 @@ static void plan_free(msosm_plan *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
-                    p->bins, p->out_u16, p->out_f16, p->out_scale};
+                    p->bins, p->out_u16, p->out_f16, p->out_scale, p->Hdot, p->tan_y};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : p->in_ev)
@@ -436,6 +448,26 @@ extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, 
     return MSOSM_OK;
 }
 
+extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
+{
+    if (!p || !dH) return set_error(MSOSM_ERR_ARG, "plan_set_dm_tangent: null argument");
+    if (i < 0 || i >= p->D) return set_error(MSOSM_ERR_ARG, "plan_set_dm_tangent: dm index out of range");
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (!p->Hdot) {
+        const size_t hb = (size_t)p->D * p->L * sizeof(float2), yb = (size_t)p->D * p->B * p->M * sizeof(float2);
+        hipError_t e = dalloc(p, &p->Hdot, hb);
+        if (e == hipSuccess) e = dalloc(p, &p->tan_y, yb);
+        if (e == hipSuccess) e = hipMemset(p->Hdot, 0, hb);
+        if (e != hipSuccess)
+            return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
+        p->tan_set.assign(p->D, 0);
+    }
+    HIPCHK(hipMemcpy(p->Hdot + (size_t)i * p->L, dH, p->L * sizeof(float2), hipMemcpyHostToDevice));
+    p->tan_set[i] = 1;
+    return MSOSM_OK;
+}
+
 static int record_begin(msosm_plan *p, EventPair &ev)
 {
     if (p->pool.empty()) {
@@ -466,30 +498,15 @@ static int collect_profile(msosm_plan *p)
     return MSOSM_OK;
 }
 
-extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
+// Arguments of every launch of one push (no launch): forward transforms of this push's blocks into
+// the ring, the overlap history for the next push, the fold bins (if a fold is attached) and the
+// dedispersion with the attached fold.
+void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
 {
-    if (!p || !input) return set_error(MSOSM_ERR_ARG, "plan_push: null argument");
-    if (mode != MSOSM_INPUT_HOST_ASYNC && mode != MSOSM_INPUT_DEVICE && mode != MSOSM_INPUT_HOST_SYNC)
-        return set_error(MSOSM_ERR_ARG, "plan_push: unknown input mode");
-    for (int i = 0; i < p->D; ++i)
-        if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
-    if (!p->fold && !p->out_flags)
-        return set_error(MSOSM_ERR_STATE, "plan_push: no output selected and no fold attached");
-    HIPCHK(hipSetDevice(p->device));
-    const size_t nsamp = (size_t)p->B * p->M;
-    const void *src = input;
-    const int slot = (int)(p->pushes & 1);
-    if (mode != MSOSM_INPUT_DEVICE) {
-        // the staging buffer was last read by the push two calls ago; wait for that to finish
-        HIPCHK(hipEventSynchronize(p->in_ev[slot]));
-        if (mode == MSOSM_INPUT_HOST_SYNC)
-            HIPCHK(hipMemcpy(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice));
-        else
-            HIPCHK(hipMemcpyAsync(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
-        src = p->in_dev[slot];
-    }
+    PushArgs &a = *out;
+    a = PushArgs{};
     const float2 *tw = p->tw;
-    FwdArgs fa{};
+    FwdArgs &fa = a.fwd;
     fa.in = src;
     fa.in_fmt = p->in_fmt;
     fa.hist = p->hist;
@@ -503,16 +520,10 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
     fa.tw_lo = tw + p->tw_lo;
     fa.tw_hi = tw + p->tw_hi;
     fa.tw_s = p->tw_s;
-    EventPair ev{};
-    if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
-    HIPCHK(launch_fwd(p->log2L, fa, std::min(p->B, p->G), p->stream));
-    if (p->profile) {
-        HIPCHK(hipEventRecord(ev.b, p->stream));
-        p->pending[0].push_back(ev);
-    }
-    HIPCHK(launch_hist(p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M, p->stream));
-    if (mode != MSOSM_INPUT_DEVICE) HIPCHK(hipEventRecord(p->in_ev[slot], p->stream));
-    InvArgs ia{};
+    a.log2L = p->log2L;
+    a.fwd_grid = std::min(p->B, p->G);
+    a.hist = HistArgs{p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M};
+    InvArgs &ia = a.inv;
     ia.ring = p->ring;
     ia.FBS = p->FBS;
     ia.g0 = p->g0;
@@ -532,41 +543,174 @@ extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode)
     ia.tw_s = p->tw_s;
     if (p->fold) {
         msosm_fold *f = p->fold;
-        HIPCHK(launch_bins(p->bins, (long long)nsamp, f->nbins, f->t_start, f->dt, f->period, p->stream));
+        a.has_bins = true;
+        a.bins = BinsArgs{p->bins, (long long)p->B * p->M, f->nbins, f->t_start, f->dt, f->period};
         ia.bins = p->bins;
         ia.fold_sum = f->sums;
         ia.fold_cnt = f->cnts;
         ia.nbins = (int)f->nbins;
     }
+}
+
+// The part of a push before the dedispersion launch: launches the forward transforms, the history
+// copy and the fold bins, and returns the dedispersion arguments (the launch is the caller's).
+int msosm::plan_push_prepare(msosm_plan *p, const void *src, InvArgs *out)
+{
+    PushArgs a;
+    plan_push_args(p, src, &a);
+    EventPair ev{};
     if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
-    if (p->launch_mode == MSOSM_LAUNCH_PER_DM) {
-        // the plain form: one launch per DM over that DM's blocks
-        for (int d = 0; d < p->D; ++d) {
-            InvArgs one = ia;
-            one.D = 1;
-            one.H = ia.H + (size_t)d * p->L;
-            one.delay = ia.delay + (size_t)d * p->L;
-            if (ia.out_c64) one.out_c64 = ia.out_c64 + (size_t)d * p->B * p->M;
-            if (ia.out_u16) one.out_u16 = ia.out_u16 + (size_t)d * p->B * p->M;
-            if (ia.out_f16) one.out_f16 = ia.out_f16 + (size_t)d * p->B * p->M;
-            if (ia.out_scale) one.out_scale = ia.out_scale + (size_t)d * p->B;
-            if (ia.fold_sum) {
-                one.fold_sum = ia.fold_sum + (size_t)d * ia.nbins;
-                one.fold_cnt = ia.fold_cnt + (size_t)d * ia.nbins;
+    HIPCHK(launch_fwd(a.log2L, a.fwd, a.fwd_grid, p->stream));
+    if (p->profile) {
+        HIPCHK(hipEventRecord(ev.b, p->stream));
+        p->pending[0].push_back(ev);
+    }
+    HIPCHK(launch_hist(a.hist.hist, a.hist.in, a.hist.fmt, a.hist.first, a.hist.M, p->stream));
+    if (a.has_bins)
+        HIPCHK(launch_bins(a.bins.bins, a.bins.n, a.bins.nbins, a.bins.t_start, a.bins.dt, a.bins.period, p->stream));
+    *out = a.inv;
+    return MSOSM_OK;
+}
+
+// The part after the dedispersion launch: advance the ring position and the fold clock.
+void msosm::plan_push_commit(msosm_plan *p)
+{
+    const size_t nsamp = (size_t)p->B * p->M;
+    p->g0 += p->B;
+    p->pushes += 1;
+    if (p->fold) p->fold->t_start += static_cast<double>(nsamp) * p->fold->dt; // fold_gpu.cpp:53
+}
+
+int msosm::plan_check_ready(const msosm_plan *p)
+{
+    for (int i = 0; i < p->D; ++i)
+        if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
+    if (!p->fold && !p->out_flags)
+        return set_error(MSOSM_ERR_STATE, "plan_push: no output selected and no fold attached");
+    return MSOSM_OK;
+}
+
+int msosm::plan_log2L(const msosm_plan *p) { return p->log2L; }
+int msosm::plan_workgroups(const msosm_plan *p) { return p->G; }
+int msosm::plan_num_units(const msosm_plan *p) { return p->D * p->B; }
+bool msosm::plan_ring32(const msosm_plan *p) { return (unsigned long long)p->FBS * p->L * sizeof(float2) < (1ull << 32); }
+int msosm::plan_launch_mode(const msosm_plan *p) { return p->launch_mode; }
+hipStream_t msosm::plan_stream(const msosm_plan *p) { return p->stream; }
+
+// One push: stage the input, forward transforms into the ring, then the dedispersion launch(es).
+// With tfold, the dedispersion runs twice: with H into the primal buffer (folding |y|^2 into the
+// attached fold, if any), then with dH/dDM, folding 2 Re(conj(y) dy/dDM) into tfold.
+static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfold)
+{
+    if (!p || !input) return set_error(MSOSM_ERR_ARG, "plan_push: null argument");
+    if (mode != MSOSM_INPUT_HOST_ASYNC && mode != MSOSM_INPUT_DEVICE && mode != MSOSM_INPUT_HOST_SYNC)
+        return set_error(MSOSM_ERR_ARG, "plan_push: unknown input mode");
+    for (int i = 0; i < p->D; ++i)
+        if (!p->dm_set[i]) return set_error(MSOSM_ERR_STATE, "plan_push: DM " + std::to_string(i) + " has no design");
+    if (!tfold && !p->fold && !p->out_flags)
+        return set_error(MSOSM_ERR_STATE, "plan_push: no output selected and no fold attached");
+    if (tfold) {
+        if (!p->Hdot) return set_error(MSOSM_ERR_STATE, "plan_push_tangent: no dH/dDM uploaded");
+        for (int i = 0; i < p->D; ++i)
+            if (!p->tan_set[i])
+                return set_error(MSOSM_ERR_STATE, "plan_push_tangent: DM " + std::to_string(i) + " has no dH/dDM");
+        if (tfold->S != p->D || tfold->device != p->device)
+            return set_error(MSOSM_ERR_ARG, "plan_push_tangent: tangent fold needs one series per DM on the plan's device");
+        if (tfold->nbins > (unsigned long)p->max_fold_bins)
+            return set_error(MSOSM_ERR_UNSUPPORTED, "plan_push_tangent: at most " + std::to_string(p->max_fold_bins) +
+                                                        " bins in LDS");
+        if (tfold->size != (unsigned long)p->B * p->M)
+            return set_error(MSOSM_ERR_ARG, "plan_push_tangent: tangent fold size must be batch*M");
+        if (p->fold && (p->fold->nbins != tfold->nbins || p->fold->period != tfold->period || p->fold->dt != tfold->dt ||
+                        p->fold->t_start != tfold->t_start))
+            return set_error(MSOSM_ERR_ARG, "plan_push_tangent: the attached fold and the tangent fold disagree on bins");
+    }
+    HIPCHK(hipSetDevice(p->device));
+    const size_t nsamp = (size_t)p->B * p->M;
+    const void *src = input;
+    const int slot = (int)(p->pushes & 1);
+    if (mode != MSOSM_INPUT_DEVICE) {
+        // the staging buffer was last read by the push two calls ago; wait for that to finish
+        HIPCHK(hipEventSynchronize(p->in_ev[slot]));
+        if (mode == MSOSM_INPUT_HOST_SYNC)
+            HIPCHK(hipMemcpy(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice));
+        else
+            HIPCHK(hipMemcpyAsync(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
+        src = p->in_dev[slot];
+    }
+    InvArgs ia{};
+    if (plan_push_prepare(p, src, &ia) != MSOSM_OK) return MSOSM_ERR_HIP;
+    if (mode != MSOSM_INPUT_DEVICE) HIPCHK(hipEventRecord(p->in_ev[slot], p->stream));
+    if (tfold && !p->fold)
+        HIPCHK(launch_bins(p->bins, (long long)nsamp, tfold->nbins, tfold->t_start, tfold->dt, tfold->period,
+                           p->stream));
+    EventPair ev{};
+    auto run_inv = [&](const InvArgs &x) -> int {
+        if (p->launch_mode == MSOSM_LAUNCH_PER_DM) {
+            // the plain form: one launch per DM over that DM's blocks
+            const size_t ser = (size_t)p->B * p->M;
+            for (int d = 0; d < p->D; ++d) {
+                InvArgs one = x;
+                one.D = 1;
+                one.H = x.H + (size_t)d * p->L;
+                one.delay = x.delay + (size_t)d * p->L;
+                if (x.out_c64) one.out_c64 = x.out_c64 + d * ser;
+                if (x.out_u16) one.out_u16 = x.out_u16 + d * ser;
+                if (x.out_f16) one.out_f16 = x.out_f16 + d * ser;
+                if (x.out_scale) one.out_scale = x.out_scale + (size_t)d * p->B;
+                if (x.tangent_ref) one.tangent_ref = x.tangent_ref + d * ser;
+                if (x.fold_sum) {
+                    one.fold_sum = x.fold_sum + (size_t)d * x.nbins;
+                    one.fold_cnt = x.fold_cnt + (size_t)d * x.nbins;
+                }
+                HIPCHK(launch_inv(p->log2L, one, std::min(p->B, p->G), p->stream));
             }
-            HIPCHK(launch_inv(p->log2L, one, std::min(p->B, p->G), p->stream));
+        } else {
+            HIPCHK(launch_inv(p->log2L, x, p->G, p->stream));
         }
+        return MSOSM_OK;
+    };
+    if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
+    if (!tfold) {
+        if (run_inv(ia) != MSOSM_OK) return MSOSM_ERR_HIP;
     } else {
-        HIPCHK(launch_inv(p->log2L, ia, p->G, p->stream));
+        // primal series into the pairing buffer (no user outputs in a tangent push), then the tangent
+        InvArgs prim = ia;
+        prim.out_c64 = p->tan_y;
+        prim.out_u16 = nullptr;
+        prim.out_f16 = nullptr;
+        prim.out_scale = nullptr;
+        if (run_inv(prim) != MSOSM_OK) return MSOSM_ERR_HIP;
+        InvArgs tan = prim;
+        tan.H = p->Hdot;
+        tan.out_c64 = nullptr;
+        tan.tangent_ref = p->tan_y;
+        tan.bins = p->bins;
+        tan.fold_sum = tfold->sums;
+        tan.fold_cnt = tfold->cnts;
+        tan.nbins = (int)tfold->nbins;
+        if (run_inv(tan) != MSOSM_OK) return MSOSM_ERR_HIP;
     }
     if (p->profile) {
         HIPCHK(hipEventRecord(ev.b, p->stream));
         p->pending[1].push_back(ev);
     }
-    p->g0 += p->B;
-    p->pushes += 1;
-    if (p->fold) p->fold->t_start += static_cast<double>(nsamp) * p->fold->dt; // fold_gpu.cpp:53
+    plan_push_commit(p);
+    if (tfold) tfold->t_start += static_cast<double>(nsamp) * tfold->dt;
     return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_push(msosm_plan *p, const void *input, int mode) { return push_impl(p, input, mode, nullptr); }
+
+extern "C" int msosm_plan_push_tangent(msosm_plan *p, const void *input, int mode, msosm_fold *tangent_fold)
+{
+    if (!p || !tangent_fold) return set_error(MSOSM_ERR_ARG, "plan_push_tangent: null argument");
+    if (!p->bins) {
+        HIPCHK(hipSetDevice(p->device));
+        hipError_t e = dalloc(p, &p->bins, (size_t)p->B * p->M * sizeof(uint16_t));
+        if (e != hipSuccess) return set_error(MSOSM_ERR_ALLOC, std::string("plan_push_tangent: ") + hipGetErrorString(e));
+    }
+    return push_impl(p, input, mode, tangent_fold);
 }
 
 extern "C" int msosm_plan_output_c64(msosm_plan *p, void **ptr)

@@ -5,14 +5,22 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "capi_internal.h"
 #include "msosm_amd.h"
+#include "plan_internal.h"
 
 using namespace msosm;
+
+#define HIPCHK_MULTI(x)                                                                                                \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+    } while (0)
 
 namespace {
 
@@ -36,7 +44,48 @@ struct msosm_multi {
     long long pushes = 0;
     hipStream_t stream = nullptr;
     unsigned long nbins = 0;
+    // one dedispersion launch over all classes (MSOSM_LAUNCH_PERSISTENT), when every class fits it
+    int launch_mode = MSOSM_LAUNCH_PERSISTENT;
+    unsigned int *counter = nullptr;
+    int max_fused_bins = 0;
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    double prof_ms = 0;
+    long long prof_n = 0;
+    // hipGraph form of the one-launch push: built on the first push, then re-parameterised per push
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> n_fwd, n_hist, n_bins;
+    hipGraphNode_t n_fill = nullptr, n_inv = nullptr;
+    std::vector<PushArgs> gargs;
+    MultiInvArgs gma{};
+    unsigned int *fill_ptr = nullptr;
+    long long fill_words = 1;
+    uint32_t fill_value = 0;
 };
+
+static void drop_graph(msosm_multi *m)
+{
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+    m->n_fwd.clear();
+    m->n_hist.clear();
+    m->n_bins.clear();
+    m->n_fill = m->n_inv = nullptr;
+}
+
+// every class can join the one-launch form: lengths 2^14..2^21, rings below 4 GiB, <= MAX_CLASSES
+static bool fused_eligible(const msosm_multi *m)
+{
+    if (m->classes.size() > (size_t)MAX_CLASSES) return false;
+    if (m->nbins > (unsigned long)m->max_fused_bins) return false;
+    for (const auto &c : m->classes)
+        if (!multi_length_supported(plan_log2L(c.plan)) || !plan_ring32(c.plan)) return false;
+    return true;
+}
 
 static void multi_free(msosm_multi *m)
 {
@@ -52,6 +101,13 @@ static void multi_free(msosm_multi *m)
     for (int i = 0; i < 2; ++i) {
         if (m->stage[i]) (void)hipFree(m->stage[i]);
         if (m->stage_ev[i]) (void)hipEventDestroy(m->stage_ev[i]);
+    }
+    drop_graph(m);
+    if (m->counter) (void)hipFree(m->counter);
+    for (auto &e : m->pending) m->pool.push_back(e);
+    for (auto &e : m->pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     delete m;
 }
@@ -131,7 +187,14 @@ extern "C" int msosm_multi_create(const msosm_multi_desc *d, msosm_multi **out)
         e = hipMalloc(&m->stage[i], (size_t)m->push * m->in_bytes);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&m->stage_ev[i], hipEventDisableTiming);
     }
+    if (e == hipSuccess) e = hipMalloc(&m->counter, sizeof(unsigned int));
+    int max_lds = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return fail(MSOSM_ERR_ALLOC, std::string("multi_create: ") + hipGetErrorString(e));
+    const int tiles = multi_tiles_lds_bytes();
+    m->max_fused_bins = std::max(0, std::min(2048, (max_lds - tiles - INV_EXTRA_LDS) / 12));
+    e = configure_multi_lds(tiles + m->max_fused_bins * 12 + INV_EXTRA_LDS);
+    if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("multi_create: LDS attribute: ") + hipGetErrorString(e));
     *out = m;
     return MSOSM_OK;
 }
@@ -167,10 +230,177 @@ extern "C" int msosm_multi_set_dm(msosm_multi *m, int dm, const int *delay_point
 extern "C" int msosm_multi_set_launch_mode(msosm_multi *m, int mode)
 {
     if (!m) return set_error(MSOSM_ERR_ARG, "multi_set_launch_mode: null object");
+    if (mode != MSOSM_LAUNCH_PERSISTENT && mode != MSOSM_LAUNCH_PER_DM && mode != MSOSM_LAUNCH_PER_CLASS)
+        return set_error(MSOSM_ERR_ARG, "multi_set_launch_mode: unknown mode");
     for (auto &c : m->classes) {
-        const int rc = msosm_plan_set_launch_mode(c.plan, mode);
+        const int rc = msosm_plan_set_launch_mode(c.plan, mode == MSOSM_LAUNCH_PER_DM ? MSOSM_LAUNCH_PER_DM
+                                                                                     : MSOSM_LAUNCH_PERSISTENT);
         if (rc != MSOSM_OK) return rc;
     }
+    m->launch_mode = mode;
+    drop_graph(m);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_multi_set_graph(msosm_multi *m, int enable)
+{
+    if (!m) return set_error(MSOSM_ERR_ARG, "multi_set_graph: null object");
+    HIPCHK_MULTI(hipStreamSynchronize(m->stream));
+    drop_graph(m);
+    m->use_graph = enable != 0;
+    return MSOSM_OK;
+}
+
+static void multi_inv_args(msosm_multi *m, MultiInvArgs &ma, const std::vector<InvArgs> &inv)
+{
+    ma = MultiInvArgs{};
+    long long acc = 0;
+    const int n = (int)m->classes.size();
+    for (int i = 0; i < n; ++i) {
+        ma.cls[i] = inv[i];
+        ma.log2L[i] = plan_log2L(m->classes[i].plan);
+        ma.ustart[i] = acc;
+        acc += plan_num_units(m->classes[i].plan);
+    }
+    ma.ustart[n] = acc;
+    ma.ncls = n;
+    ma.nbins = ma.cls[0].fold_sum ? ma.cls[0].nbins : 0;
+    // class 0 has the longest transforms; its workgroup scratch slots hold any class's
+    ma.scratch = ma.cls[0].scratch;
+    ma.scr_stride = 1LL << ma.log2L[0];
+    ma.counter = m->counter;
+}
+
+static hipKernelNodeParams node_params(const NodeDesc &d, void **args)
+{
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(d.func);
+    p.gridDim = d.grid;
+    p.blockDim = d.block;
+    p.sharedMemBytes = d.lds;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return p;
+}
+
+// The one-launch push as a hipGraph: [counter fill] + per class [forward -> history, bins] ->
+// multi-length dedispersion. The graph is built once; later pushes only update the node arguments
+// (ring positions, input address, fold clock) and launch it.
+static int graph_push(msosm_multi *m, const void *src)
+{
+    const int n = (int)m->classes.size();
+    m->gargs.resize(n);
+    std::vector<InvArgs> inv(n);
+    for (int i = 0; i < n; ++i) {
+        plan_push_args(m->classes[i].plan, src, &m->gargs[i]);
+        inv[i] = m->gargs[i].inv;
+    }
+    multi_inv_args(m, m->gma, inv);
+    m->fill_ptr = m->counter;
+    m->fill_words = 1;
+    m->fill_value = 0;
+    const int grid = plan_workgroups(m->classes[0].plan);
+    // argument pointer lists (hipGraph*KernelNode* copy the values they point to)
+    std::vector<std::array<void *, 6>> pf(n), ph(n), pb(n);
+    for (int i = 0; i < n; ++i) {
+        PushArgs &a = m->gargs[i];
+        pf[i] = {&a.fwd, nullptr, nullptr, nullptr, nullptr, nullptr};
+        ph[i] = {&a.hist.hist, &a.hist.in, &a.hist.fmt, &a.hist.first, &a.hist.M, nullptr};
+        pb[i] = {&a.bins.bins, &a.bins.n, &a.bins.nbins, &a.bins.t_start, &a.bins.dt, &a.bins.period};
+    }
+    void *pfill[3] = {&m->fill_ptr, &m->fill_words, &m->fill_value};
+    void *pinv[1] = {&m->gma};
+    const NodeDesc dfill = fill_node(4), dinv = inv_multi_node(m->gma, grid);
+    if (!m->exec) {
+        HIPCHK_MULTI(hipGraphCreate(&m->graph, 0));
+        hipKernelNodeParams kp = node_params(dfill, pfill);
+        HIPCHK_MULTI(hipGraphAddKernelNode(&m->n_fill, m->graph, nullptr, 0, &kp));
+        std::vector<hipGraphNode_t> deps{m->n_fill};
+        m->n_fwd.assign(n, nullptr);
+        m->n_hist.assign(n, nullptr);
+        m->n_bins.assign(n, nullptr);
+        for (int i = 0; i < n; ++i) {
+            const PushArgs &a = m->gargs[i];
+            kp = node_params(fwd_node(a.log2L, a.fwd_grid), pf[i].data());
+            HIPCHK_MULTI(hipGraphAddKernelNode(&m->n_fwd[i], m->graph, nullptr, 0, &kp));
+            kp = node_params(hist_node(a.hist.M), ph[i].data());
+            HIPCHK_MULTI(hipGraphAddKernelNode(&m->n_hist[i], m->graph, &m->n_fwd[i], 1, &kp));
+            deps.push_back(m->n_fwd[i]);
+            if (a.has_bins) {
+                kp = node_params(bins_node(a.bins.n), pb[i].data());
+                HIPCHK_MULTI(hipGraphAddKernelNode(&m->n_bins[i], m->graph, nullptr, 0, &kp));
+                deps.push_back(m->n_bins[i]);
+            }
+        }
+        kp = node_params(dinv, pinv);
+        HIPCHK_MULTI(hipGraphAddKernelNode(&m->n_inv, m->graph, deps.data(), deps.size(), &kp));
+        HIPCHK_MULTI(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+    } else {
+        for (int i = 0; i < n; ++i) {
+            const PushArgs &a = m->gargs[i];
+            hipKernelNodeParams kp = node_params(fwd_node(a.log2L, a.fwd_grid), pf[i].data());
+            HIPCHK_MULTI(hipGraphExecKernelNodeSetParams(m->exec, m->n_fwd[i], &kp));
+            kp = node_params(hist_node(a.hist.M), ph[i].data());
+            HIPCHK_MULTI(hipGraphExecKernelNodeSetParams(m->exec, m->n_hist[i], &kp));
+            if (a.has_bins) {
+                kp = node_params(bins_node(a.bins.n), pb[i].data());
+                HIPCHK_MULTI(hipGraphExecKernelNodeSetParams(m->exec, m->n_bins[i], &kp));
+            }
+        }
+        hipKernelNodeParams kp = node_params(dinv, pinv);
+        HIPCHK_MULTI(hipGraphExecKernelNodeSetParams(m->exec, m->n_inv, &kp));
+    }
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (m->profile) {
+        if (m->pool.empty()) {
+            HIPCHK_MULTI(hipEventCreate(&ev.first));
+            HIPCHK_MULTI(hipEventCreate(&ev.second));
+        } else {
+            ev = m->pool.back();
+            m->pool.pop_back();
+        }
+        HIPCHK_MULTI(hipEventRecord(ev.first, m->stream));
+    }
+    HIPCHK_MULTI(hipGraphLaunch(m->exec, m->stream));
+    if (m->profile) {
+        HIPCHK_MULTI(hipEventRecord(ev.second, m->stream));
+        m->pending.push_back(ev);
+    }
+    for (auto &c : m->classes) plan_push_commit(c.plan);
+    return MSOSM_OK;
+}
+
+// One dedispersion launch over every class: the forward parts of all class pushes, then the
+// multi-length persistent kernel, then each class's ring/fold advance.
+static int fused_push(msosm_multi *m, const void *src)
+{
+    const int n = (int)m->classes.size();
+    std::vector<InvArgs> inv(n);
+    for (int i = 0; i < n; ++i) {
+        const int rc = plan_push_prepare(m->classes[i].plan, src, &inv[i]);
+        if (rc != MSOSM_OK) return rc;
+    }
+    MultiInvArgs ma;
+    multi_inv_args(m, ma, inv);
+    hipError_t e = launch_fill(m->counter, sizeof(unsigned int), 0u, m->stream);
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (e == hipSuccess && m->profile) {
+        if (m->pool.empty()) {
+            e = hipEventCreate(&ev.first);
+            if (e == hipSuccess) e = hipEventCreate(&ev.second);
+        } else {
+            ev = m->pool.back();
+            m->pool.pop_back();
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev.first, m->stream);
+    }
+    if (e == hipSuccess) e = launch_inv_multi(ma, plan_workgroups(m->classes[0].plan), m->stream);
+    if (e == hipSuccess && m->profile) {
+        e = hipEventRecord(ev.second, m->stream);
+        m->pending.push_back(ev);
+    }
+    if (e != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string("multi_push: ") + hipGetErrorString(e));
+    for (auto &c : m->classes) plan_push_commit(c.plan);
     return MSOSM_OK;
 }
 
@@ -191,9 +421,16 @@ extern "C" int msosm_multi_push(msosm_multi *m, const void *input, int mode)
         if (e != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string("multi_push: ") + hipGetErrorString(e));
         src = m->stage[slot];
     }
-    for (auto &c : m->classes) {
-        const int rc = msosm_plan_push(c.plan, src, MSOSM_INPUT_DEVICE);
+    if (m->launch_mode == MSOSM_LAUNCH_PERSISTENT && fused_eligible(m)) {
+        for (auto &c : m->classes)
+            if (plan_check_ready(c.plan) != MSOSM_OK) return MSOSM_ERR_STATE;
+        const int rc = m->use_graph ? graph_push(m, src) : fused_push(m, src);
         if (rc != MSOSM_OK) return rc;
+    } else {
+        for (auto &c : m->classes) {
+            const int rc = msosm_plan_push(c.plan, src, MSOSM_INPUT_DEVICE);
+            if (rc != MSOSM_OK) return rc;
+        }
     }
     if (mode != MSOSM_INPUT_DEVICE) {
         hipError_t e = hipEventRecord(m->stage_ev[slot], m->stream);
@@ -227,6 +464,8 @@ extern "C" int msosm_multi_get_output(msosm_multi *m, int dm, void *host)
 extern "C" int msosm_multi_fold(msosm_multi *m, double period, double fs, unsigned long nbins)
 {
     if (!m) return set_error(MSOSM_ERR_ARG, "multi_fold: null object");
+    HIPCHK_MULTI(hipStreamSynchronize(m->stream));
+    drop_graph(m);
     for (auto &c : m->classes) {
         if (c.fold) {
             msosm_plan_attach_fold(c.plan, nullptr);
@@ -268,9 +507,28 @@ extern "C" int msosm_multi_synchronize(msosm_multi *m)
     return msosm_plan_synchronize(m->classes.front().plan);
 }
 
+static int collect_fused(msosm_multi *m)
+{
+    for (auto &ev : m->pending) {
+        float ms = 0;
+        hipError_t e = hipEventSynchronize(ev.second);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev.first, ev.second);
+        if (e != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string("multi_profile: ") + hipGetErrorString(e));
+        m->prof_ms += ms;
+        m->prof_n += 1;
+        m->pool.push_back(ev);
+    }
+    m->pending.clear();
+    return MSOSM_OK;
+}
+
 extern "C" int msosm_multi_profile(msosm_multi *m, int enable)
 {
     if (!m) return set_error(MSOSM_ERR_ARG, "multi_profile: null object");
+    if (collect_fused(m) != MSOSM_OK) return MSOSM_ERR_HIP;
+    m->profile = enable != 0;
+    m->prof_ms = 0;
+    m->prof_n = 0;
     for (auto &c : m->classes) {
         const int rc = msosm_plan_profile(c.plan, enable);
         if (rc != MSOSM_OK) return rc;
@@ -290,6 +548,11 @@ extern "C" int msosm_multi_profile_read(msosm_multi *m, int kernel, double *tota
         if (rc != MSOSM_OK) return rc;
         ms += a;
         n += b;
+    }
+    if (kernel == 1) {
+        if (collect_fused(m) != MSOSM_OK) return MSOSM_ERR_HIP;
+        ms += m->prof_ms;
+        n += m->prof_n;
     }
     if (total_ms) *total_ms = ms;
     if (launches) *launches = n;

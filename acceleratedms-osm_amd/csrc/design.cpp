@@ -183,7 +183,7 @@ int design_build_segments(const int *delay, int fft_len, int max_len, int *seg, 
 
 // Chirp sample n of the M-point design grid: c_n = e^{-i phi(f_n)}, f_n = ((n + M/2) mod M) fs/M
 // above the band bottom (msosm_prepare.cpp:122-138; legacy float-w form at :179-192).
-static std::complex<double> chirp(float bw, float f0, float dm, int M, int n, int phase_mode)
+static double chirp_phase(float bw, float f0, float dm, int M, int n, int phase_mode)
 {
     const int j = (n < M / 2) ? n + M / 2 : n - M / 2; // grid index of bin n
     double phase;
@@ -201,16 +201,33 @@ static std::complex<double> chirp(float bw, float f0, float dm, int M, int n, in
         phase = 4 * kPi * kPi * (double)kdm * (double)dm * (double)w * (double)w / (double)(w + w0) / (double)w0 /
                 (double)w0;
     }
+    return phase;
+}
+
+static std::complex<double> chirp(float bw, float f0, float dm, int M, int n, int phase_mode)
+{
+    const double phase = chirp_phase(bw, f0, dm, M, n, phase_mode);
     return {std::cos(phase), -std::sin(phase)};
+}
+
+// d c_n / d DM = -i (phi_n / DM) c_n: the phase is linear in DM, so phi_n / DM is the phase at DM = 1
+static std::complex<double> chirp_ddm(float bw, float f0, float dm, int M, int n, int phase_mode)
+{
+    const double unit = chirp_phase(bw, f0, 1.0f, M, n, phase_mode);
+    const std::complex<double> c = chirp(bw, f0, dm, M, n, phase_mode);
+    return std::complex<double>(0.0, -unit) * c;
 }
 
 // H = DFT_2M([IDFT_M(c)/M, 0]) (msosm_prepare.cpp:140-161), evaluated in FP64 as
 //   H[2n] = c_n, H[2n+1] = DFT_M(e^{-i pi j/M} IDFT_M(c)[j] / M)[n],
 // then rounded once to complex64.
-int design_transfer_function(float bw, float f0, float dm, int M, int phase_mode, float *H, const Fft64Handle *fft)
+// With deriv != 0 the same map is applied to d c / d DM, which gives d H / d DM (the map is linear).
+int design_transfer_function(float bw, float f0, float dm, int M, int phase_mode, float *H, const Fft64Handle *fft,
+                             int deriv)
 {
     std::vector<std::complex<double>> c(M), h(M);
-    for (int n = 0; n < M; ++n) c[n] = chirp(bw, f0, dm, M, n, phase_mode);
+    for (int n = 0; n < M; ++n)
+        c[n] = deriv ? chirp_ddm(bw, f0, dm, M, n, phase_mode) : chirp(bw, f0, dm, M, n, phase_mode);
     h = c;
     const Fft64 *f = reinterpret_cast<const Fft64 *>(fft);
     Fft64 local(fft ? 1 : M);
@@ -238,7 +255,7 @@ const Fft64Handle *design_fft_create(int M) { return reinterpret_cast<const Fft6
 void design_fft_destroy(const Fft64Handle *h) { delete reinterpret_cast<const Fft64 *>(h); }
 
 int design_transfer_functions(float bw, float f0, const float *dms, int num_dms, int M, int phase_mode, float *H,
-                              int nthreads)
+                              int nthreads, int deriv)
 {
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
     nthreads = std::min(nthreads, num_dms);
@@ -247,7 +264,7 @@ int design_transfer_functions(float bw, float f0, const float *dms, int num_dms,
     for (int t = 0; t < nthreads; ++t)
         pool.emplace_back([&, t] {
             for (int i = t; i < num_dms; i += nthreads)
-                design_transfer_function(bw, f0, dms[i], M, phase_mode, H + 4 * (size_t)M * i, fft);
+                design_transfer_function(bw, f0, dms[i], M, phase_mode, H + 4 * (size_t)M * i, fft, deriv);
         });
     for (auto &th : pool) th.join();
     design_fft_destroy(fft);

@@ -33,6 +33,7 @@ struct InvArgs {
     uint32_t *out_u16;        // [D][B][M] or null
     uint32_t *out_f16;        // [D][B][M] FP16 pairs or null
     float *out_scale;         // [D][B] scales of out_f16
+    const float2 *tangent_ref; // [D][B][M] primal series: fold 2 Re(conj(ref) y) instead of |y|^2, or null
     const uint16_t *bins;     // [B*M] fold bin of each call-relative sample, or null
     double *fold_sum;         // [D][nbins] or null (no fold)
     unsigned long long *fold_cnt;
@@ -40,6 +41,38 @@ struct InvArgs {
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
 };
+
+// One persistent launch over several FFT lengths ("length classes"): class c owns the units
+// [ustart[c], ustart[c+1]) (unit = (DM, block) of that class, block fastest); workgroups take units
+// in increasing order from a device counter, so the long transforms (class 0 = largest) start first.
+constexpr int MAX_CLASSES = 8;
+struct MultiInvArgs {
+    InvArgs cls[MAX_CLASSES];  // per class, as for one-length launches (rings below 4 GiB)
+    int log2L[MAX_CLASSES];
+    long long ustart[MAX_CLASSES + 1];
+    int ncls;
+    int nbins;                 // common fold bins of every class (0 = no fold)
+    float2 *scratch;           // [grid][scr_stride] workgroup scratch slots
+    long long scr_stride;      // >= the largest L
+    unsigned int *counter;     // unit counter, zero at launch
+};
+// A kernel launch as a graph node: function, launch shape and dynamic LDS (the arguments are the
+// launcher's, in order).
+struct NodeDesc {
+    const void *func = nullptr;
+    dim3 grid, block;
+    unsigned int lds = 0;
+};
+NodeDesc fwd_node(int log2L, int grid);        // fwd_kernel(FwdArgs)
+NodeDesc hist_node(int M);                     // hist_kernel(hist, in, fmt, first, M)
+NodeDesc bins_node(long long n);               // bins_kernel(bins, n, nbins, t_start, dt, period)
+NodeDesc fill_node(long long nbytes);          // fill_kernel(words, nwords, value)
+NodeDesc inv_multi_node(const MultiInvArgs &a, int grid); // inv_multi_kernel(MultiInvArgs)
+
+bool multi_length_supported(int log2L);
+int multi_tiles_lds_bytes();
+hipError_t configure_multi_lds(int bytes);
+hipError_t launch_inv_multi(const MultiInvArgs &a, int grid, hipStream_t st);
 
 void split_length(int log2L, int *n1, int *n2);
 bool length_supported(int log2L);

@@ -1,0 +1,382 @@
+// Device building blocks of the MS-OSM kernels for gfx950: the four-step FFT passes of one
+// 1024-thread workgroup, input decoding, output codecs and the dedispersion unit.
+//
+// Every transform of length L = N1 * N2 is a four-step FFT:
+//   pass A: N1 column sub-FFTs of length N2 (stride-N1 elements), fused with the producer (input
+//           unpack, or the per-bin ring gather times the chirp H), then the W_L^{n2 k1} twiddle,
+//           into the workgroup's private scratch slot;
+//   pass B: N2 row sub-FFTs of length N1 from the scratch slot, fused with the consumer (ring
+//           store, or discard + 1/L scale + fold / complex64 / uint16 / FP16 output).
+// A workgroup keeps four 4096-point tiles in flight (256 threads each).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_engine.h"
+#include "kernels.h"
+
+namespace msosm {
+
+constexpr int TILE = 4096;
+constexpr int SUBT = 4;
+constexpr int WG = 256 * SUBT;
+
+template <int NS>
+struct TileWords {
+    // LDS float2 words one tile needs in either pass layout and in the padded sample-order buffer
+    static constexpr int NC = TILE / NS;
+    static constexpr int A = TileLDS<NS, NC, true>::ELEMS;
+    static constexpr int B = TileLDS<NS, NC, false>::ELEMS;
+    static constexpr int T = TILE + TILE / 16;
+    static constexpr int AB = A > B ? A : B;
+    static constexpr int V = AB > T ? AB : T;
+};
+
+template <int N1, int N2>
+struct Shape {
+    static constexpr int L = N1 * N2;
+    static constexpr int NCA = TILE / N2 < 1 ? 1 : TILE / N2; // columns per pass-A tile
+    static constexpr int NCB = TILE / N1 < 1 ? 1 : TILE / N1; // rows per pass-B tile
+    static constexpr int NTA = N1 / NCA;                      // pass-A tiles
+    static constexpr int NTB = N2 / NCB;                      // pass-B tiles
+    static constexpr int WA = TileWords<N2>::V, WB = TileWords<N1>::V;
+    static constexpr int TILE_WORDS = WA > WB ? WA : WB;
+    static constexpr int LDS_TILES_BYTES = SUBT * TILE_WORDS * 8;
+    // sub-FFT twiddle tables for N1 and N2 follow the tiles when they are small (<= 8 KiB; the
+    // largest lengths read them from global memory); the fold histogram comes last
+    static constexpr bool TW_IN_LDS = false; // measured: global (L1-resident) tables are faster here
+    static constexpr int LDS_TW_BYTES = TW_IN_LDS ? (N1 + N2) * 8 : 0;
+    static constexpr int LDS_BYTES = LDS_TILES_BYTES + LDS_TW_BYTES;
+};
+
+
+
+// Element idx of an array whose byte size is below 4 GiB: a 32-bit byte offset from a (uniform)
+// base lets the compiler use the SGPR-base + 32-bit VGPR-offset addressing form, with no per-lane
+// 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, uint32_t idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T *base, uint32_t idx, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+
+struct Tw {
+    const float2 *n1;  // e^{2 pi i m / N1}, m < N1
+    const float2 *n2;  // e^{2 pi i m / N2}, m < N2
+    const float2 *lo;  // e^{2 pi i m / L}, m < 2^s
+    const float2 *hi;  // e^{2 pi i m 2^s / L}, m < L / 2^s
+    int s;
+};
+
+// copy the N1- and N2-point twiddle tables into LDS (read once per kernel)
+template <int N1, int N2>
+__device__ __forceinline__ void load_twiddles(float2 *lds_all, Tw &tw)
+{
+    using S = Shape<N1, N2>;
+    if constexpr (!S::TW_IN_LDS) return;
+    float2 *t1 = lds_all + S::LDS_TILES_BYTES / 8;
+    float2 *t2 = t1 + N1;
+    for (int i = threadIdx.x; i < N1; i += blockDim.x) t1[i] = tw.n1[i];
+    for (int i = threadIdx.x; i < N2; i += blockDim.x) t2[i] = tw.n2[i];
+    tw.n1 = t1;
+    tw.n2 = t2;
+    __syncthreads();
+}
+
+template <int DIR>
+__device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
+{
+    float2 w = cmul(tw.hi[m >> tw.s], tw.lo[m & ((1 << tw.s) - 1)]);
+    return DIR < 0 ? cconj(w) : w;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pass A: column sub-FFTs. load(k) returns input element k (k = k1 + N1*k2).
+// Output Z[k1][n2] * W^{DIR n2 k1} goes to scr[(tile * N2 + n2) * NCA + col].
+// ---------------------------------------------------------------------------------------------
+template <int N1, int N2, int DIR, class Load>
+__device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ scr, float2 *lds_all, const Tw &tw)
+{
+    using S = Shape<N1, N2>;
+    constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
+#pragma unroll 1
+    for (int t0 = 0; t0 < S::NTA; t0 += SUBT) {
+        // opaque thread index: keeps per-thread addresses from being hoisted (and spilled) across tiles
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int col = lt % NC, jt = lt / NC;
+        float2 *lds = lds_all + sub * S::TILE_WORDS;
+        const int tile = t0 + sub;
+        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA; // all sub-tiles busy when divisible
+        const int k1 = tile * NC + col;
+        float2 v[16];
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = load(k1 + N1 * (jt + t * TPC));
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = make_float2(0.f, 0.f);
+        }
+        subfft<NS, NC, true, DIR>(v, lds, col, jt, tw.n2);
+        if (active) {
+            // element (q, t) is output n2 = jt + q*TPC + t*NsL; its twiddle W^{n2 k1} is
+            // W^{(jt + q*TPC) k1} (table) times c^t with c = W^{NsL k1} from binary powers, so a
+            // thread does 2 + 2*NB table reads per tile instead of two per element
+            constexpr int R = FinalShape<NS>::R, NsL = NS / R;
+            const float2 c = big_twiddle<DIR>(tw, k1 * NsL);
+#pragma unroll
+            for (int q = 0; q < 16 / R; ++q) {
+                float2 w = big_twiddle<DIR>(tw, (jt + q * TPC) * k1);
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const int n2 = final_pos<NS>(jt, q, t);
+                    st32(scr, (uint32_t)((tile * N2 + n2) * NC + col), cmul(v[q * R + t], w));
+                    if (t + 1 < R) w = cmul(w, c); // depth <= 15 products: < 1e-6 relative
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pass B: row sub-FFTs of the scratch slot. For each tile, emit(lds, tile, active) is called with
+// the tile's outputs in LDS in padded sample order: lds[lds_pad(f)] = y[n2 + N2 * n1] for
+// f = n1 * NCB + row, n2 = tile * NCB + row. Epilogues read f = lane + 256 * i, so consecutive
+// lanes hold consecutive samples (runs of NCB) and neither the LDS reads nor the stores conflict.
+// ---------------------------------------------------------------------------------------------
+template <int N1, int N2, int DIR, class Emit>
+__device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emit &emit, float2 *lds_all, const Tw &tw)
+{
+    using S = Shape<N1, N2>;
+    constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
+#pragma unroll 1
+    for (int t0 = 0; t0 < S::NTB; t0 += SUBT) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int jt = lt % TPC, row = lt / TPC;
+        float2 *lds = lds_all + sub * S::TILE_WORDS;
+        const int tile = t0 + sub;
+        const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
+        const int n2 = tile * NC + row;
+        float2 v[16];
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int k1 = jt + t * TPC;
+                v[t] = ld32(scr, (uint32_t)(((k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = make_float2(0.f, 0.f);
+        }
+        subfft<NS, NC, false, DIR>(v, lds, row, jt, tw.n1);
+        __syncthreads();
+        {
+            constexpr int R = FinalShape<NS>::R;
+#pragma unroll
+            for (int q = 0; q < 16 / R; ++q)
+#pragma unroll
+                for (int t = 0; t < R; ++t) lds[lds_pad(final_pos<NS>(jt, q, t) * NC + row)] = v[q * R + t];
+        }
+        __syncthreads();
+        emit(lds, tile, active);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Input decoding
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 decode_in(const void *in, int fmt, long long s)
+{
+    // one push is far below 4 GiB, so the sample offset is a 32-bit byte offset
+    if (fmt == MSOSM_IN_U16) {
+        // uint16 pair, offset binary with code 0 reserved for 0 (gpu_kernel.cu:22-29)
+        const uint32_t u = ld32(reinterpret_cast<const uint32_t *>(in), (uint32_t)s);
+        const uint32_t a = u & 0xFFFFu, b = u >> 16;
+        return make_float2((float)a - (a ? 32768.f : 0.f), (float)b - (b ? 32768.f : 0.f));
+    } else if (fmt == MSOSM_IN_I8) {
+        const char2 c = ld32(reinterpret_cast<const char2 *>(in), (uint32_t)s);
+        return make_float2((float)c.x, (float)c.y);
+    }
+    return ld32(reinterpret_cast<const float2 *>(in), (uint32_t)s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dedispersion: gather * H -> IDFT_L -> keep [M, L) * 1/L -> fold / complex64 / uint16.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t enc_u16(float x)
+{
+    // round-to-nearest-even of x + 32768*[x != 0], clamped to [0, 65535] (NaN -> 0)
+    float v = x + (x != 0.f ? 32768.f : 0.f);
+    v = fminf(fmaxf(v, 0.f), 65535.f);
+    return (uint32_t)rintf(v);
+}
+
+// FP16 pair of y / scale (exact power-of-two scaling, round-to-nearest-even, saturated at +-65504)
+__device__ __forceinline__ uint32_t enc_f16_pair(float2 y, float inv_scale)
+{
+    const float xr = fminf(fmaxf(y.x * inv_scale, -65504.f), 65504.f);
+    const float xi = fminf(fmaxf(y.y * inv_scale, -65504.f), 65504.f);
+    const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, hr) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+}
+
+// zero the LDS fold histogram and the FP16-scale word (made visible by the first barrier of a unit)
+__device__ __forceinline__ void fold_lds_init(double *hsum, unsigned int *hcnt, float *pw_lds, int nbins)
+{
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+        hsum[i] = 0.0;
+        hcnt[i] = 0u;
+    }
+    if (threadIdx.x == 0) *pw_lds = 0.f;
+}
+
+// Epilogue variants of the dedispersion unit, fixed at compile time so that the plain path carries
+// no per-sample tests: VAR_F16 adds the FP16-pair output and its block scale, VAR_TANGENT folds
+// 2 Re(conj(ref) y) against a primal series instead of |y|^2.
+constexpr int VAR_PLAIN = 0, VAR_F16 = 1, VAR_TANGENT = 2;
+
+// One dedispersion unit (DM d, block b): gather * H -> IDFT_L -> keep [M, L) * 1/L -> outputs and
+// fold. hsum/hcnt are the workgroup's LDS fold histogram (zero on entry and on exit), pw_lds the
+// FP16-scale reduction word (zero on entry and on exit).
+template <int N1, int N2, bool RING32, int VAR>
+__device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
+                                         float2 *__restrict__ scr, double *hsum, unsigned int *hcnt, float *pw_lds)
+{
+    using S = Shape<N1, N2>;
+    constexpr int L = S::L, M = L / 2;
+    const bool fold = a.fold_sum != nullptr;
+    constexpr bool f16 = (VAR & VAR_F16) != 0;
+    constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
+    const float scale = 1.0f / (float)L;
+    // opaque thread index: keeps thread-dependent values from being hoisted out of the caller's
+    // unit loop (where they would stay live, and spill, across units)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int gm = (int)((a.g0 + b) % a.FBS); // ring slot of this block
+    const int FBS = a.FBS;
+    const float2 *__restrict__ Hd = a.H + (size_t)d * L;
+    const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
+    const float2 *__restrict__ ring = a.ring;
+    float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval)
+    auto load = [&](int k) -> float2 {
+        int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
+        if (src < 0) src += FBS;
+        const float2 x = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + (uint32_t)k) : ring[(size_t)src * L + k];
+        const float2 y = cmul(x, ld32(Hd, (uint32_t)k));
+        if constexpr (f16) power += y.x * y.x + y.y * y.y;
+        return y;
+    };
+    pass_a<N1, N2, +1>(load, scr, lds_all, tw);
+    float inv_scale = 1.f, out_scale = 1.f;
+    if constexpr (f16) {
+        // block rms of the 1/L-scaled series: sqrt(sum |Y|^2) / L; scale = 2^floor(log2(rms))
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
+        if ((tid & 63) == 0) atomicAdd(pw_lds, power);
+        __syncthreads();
+        const float tot = *pw_lds;
+        const float rms = sqrtf(tot) * scale;
+        int e2 = 0;
+        (void)frexpf(rms, &e2); // rms = m * 2^e2, m in [0.5, 1)
+        out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
+        inv_scale = 1.f / out_scale; // exact: a power of two
+        if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
+    }
+    __syncthreads();
+    float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * M : nullptr;
+    uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * M : nullptr;
+    uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
+    const uint16_t *bins = a.bins ? a.bins + (size_t)b * M : nullptr;
+    const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * M : nullptr;
+    auto emit = [&](const float2 *lds, int tile, bool active) {
+        if (!active) return;
+        constexpr int NC = S::NCB;
+        const int lt = tid & 255;
+        // upper half of the tile's outputs (n1 >= N1/2): 8 consecutive f per thread, i.e. runs
+        // of consecutive samples; the fold accumulates a run in registers and flushes on a bin change
+        const int fbase = (N1 / 2) * NC + lt * 8;
+        int cur = -1;
+        double csum = 0.0;
+        unsigned int ccnt = 0;
+        // runs of >= 8 consecutive samples: their 8 bins are one aligned 16-byte load
+        uint16_t bv[8];
+        if (fold && NC >= 8) {
+            const int n1 = fbase / NC, row = fbase % NC;
+            const uint4 b4 = *reinterpret_cast<const uint4 *>(bins + (tile * NC + row + N2 * n1 - M));
+            const uint32_t w4[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                bv[2 * i] = (uint16_t)(w4[i] & 0xFFFFu);
+                bv[2 * i + 1] = (uint16_t)(w4[i] >> 16);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int f = fbase + i;
+            const int n1 = f / NC, row = f % NC;
+            const int m = tile * NC + row + N2 * n1 - M;
+            float2 y = lds[lds_pad(f)];
+            y.x *= scale;
+            y.y *= scale;
+            if (oc) st32(oc, (uint32_t)m, y);
+            if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
+            if constexpr (f16)
+                if (oh) st32(oh, (uint32_t)m, enc_f16_pair(y, inv_scale));
+            if (fold) {
+                // primal: I = |y|^2. Tangent (y = d series / d DM, ref = series): the sample's
+                // contribution is dI/dDM = 2 Re(conj(ref) y), kept or skipped as the primal was
+                float I = y.x * y.x + y.y * y.y, v = I;
+                if constexpr (tangent) {
+                    const float2 r = ld32(tref, (uint32_t)m);
+                    I = r.x * r.x + r.y * r.y;
+                    v = 2.f * (r.x * y.x + r.y * y.y);
+                }
+                if (I != 0.0f) {
+                    const int bin = NC >= 8 ? (int)bv[i] : (int)bins[m];
+                    if (bin != cur) {
+                        if (ccnt) {
+                            atomicAdd(&hsum[cur], csum);
+                            atomicAdd(&hcnt[cur], ccnt);
+                        }
+                        cur = bin;
+                        csum = 0.0;
+                        ccnt = 0;
+                    }
+                    csum += (double)v;
+                    ccnt += 1;
+                }
+            }
+        }
+        if (fold && ccnt) {
+            atomicAdd(&hsum[cur], csum);
+            atomicAdd(&hcnt[cur], ccnt);
+        }
+    };
+    pass_b<N1, N2, +1>(scr, emit, lds_all, tw);
+    __syncthreads();
+    if (f16 && tid == 0) *pw_lds = 0.f; // next unit's reduction (ordered by the barriers)
+    if (fold) {
+        double *gs = a.fold_sum + (size_t)d * a.nbins;
+        unsigned long long *gc = a.fold_cnt + (size_t)d * a.nbins;
+        for (int i = tid; i < a.nbins; i += WG) {
+            const unsigned int c = hcnt[i];
+            if (c) {
+                unsafeAtomicAdd(&gs[i], hsum[i]);
+                atomicAdd(&gc[i], (unsigned long long)c);
+                hsum[i] = 0.0;
+                hcnt[i] = 0u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace msosm

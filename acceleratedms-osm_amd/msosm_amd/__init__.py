@@ -37,8 +37,10 @@ EXPORTS = [
     "msosm_multi_set_dm", "msosm_multi_set_launch_mode", "msosm_multi_push", "msosm_multi_output_c64",
     "msosm_multi_get_output", "msosm_multi_fold", "msosm_multi_fold_read", "msosm_multi_synchronize",
     "msosm_multi_profile", "msosm_multi_profile_read",
+    "msosm_design_transfer_function_ddm", "msosm_plan_set_dm_tangent", "msosm_plan_push_tangent",
+    "msosm_multi_set_graph",
 ]
-LAUNCH_PERSISTENT, LAUNCH_PER_DM = 0, 1
+LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 
 
 class MsosmError(RuntimeError):
@@ -129,6 +131,10 @@ def lib():
         "msosm_multi_synchronize": ([vp], C.c_int),
         "msosm_multi_profile": ([vp, C.c_int], C.c_int),
         "msosm_multi_profile_read": ([vp, C.c_int, dp, C.POINTER(C.c_longlong)], C.c_int),
+        "msosm_design_transfer_function_ddm": ([C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, fp], C.c_int),
+        "msosm_plan_set_dm_tangent": ([vp, C.c_int, fp], C.c_int),
+        "msosm_plan_push_tangent": ([vp, vp, C.c_int, vp], C.c_int),
+        "msosm_multi_set_graph": ([vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -198,6 +204,13 @@ def design_transfer_function(bw: float, f0: float, dm: float, M: int, phase_mode
     H = np.empty(2 * M, dtype=np.complex64)
     _check(lib().msosm_design_transfer_function(bw, f0, dm, M, phase_mode, _fp(H.view(np.float32))))
     return H
+
+
+def design_transfer_function_ddm(bw: float, f0: float, dm: float, M: int, phase_mode: int = PHASE_FP64) -> np.ndarray:
+    """dH/dDM (2M complex64): the analytic DM derivative of the transfer function."""
+    dH = np.empty(2 * M, dtype=np.complex64)
+    _check(lib().msosm_design_transfer_function_ddm(bw, f0, dm, M, phase_mode, _fp(dH.view(np.float32))))
+    return dH
 
 
 def design_transfer_functions(bw: float, f0: float, dms, M: int, phase_mode: int = PHASE_FP64,
@@ -372,6 +385,24 @@ class Plan:
         _check(lib().msosm_plan_attach_fold(self.h, fold.h if fold else None))
         self.fold = fold
 
+    def set_dm_tangent(self, i: int, dH: np.ndarray) -> None:
+        """Upload dH/dDM of DM i for tangent pushes."""
+        h = np.ascontiguousarray(dH, dtype=np.complex64)
+        if h.size != 2 * self.M:
+            raise ValueError("set_dm_tangent: dH must have 2M entries")
+        _check(lib().msosm_plan_set_dm_tangent(self.h, i, _fp(h.view(np.float32))))
+
+    def push_tangent(self, x, tangent_fold: Fold) -> None:
+        """Push batch*M samples, folding |y|^2 into the attached fold (if any) and d|y|^2/dDM into
+        tangent_fold."""
+        if isinstance(x, int):
+            _check(lib().msosm_plan_push_tangent(self.h, C.c_void_p(x), INPUT_DEVICE, tangent_fold.h))
+            return
+        a = np.ascontiguousarray(x)
+        if a.nbytes != self.B * self.M * IN_BYTES[self.in_format]:
+            raise ValueError(f"push_tangent: expected {self.B * self.M} samples, got {a.nbytes} bytes")
+        _check(lib().msosm_plan_push_tangent(self.h, C.c_void_p(a.ctypes.data), INPUT_HOST_SYNC, tangent_fold.h))
+
     def synchronize(self):
         _check(lib().msosm_plan_synchronize(self.h))
 
@@ -435,6 +466,9 @@ class Multi:
 
     def set_launch_mode(self, mode: int):
         _check(lib().msosm_multi_set_launch_mode(self.h, mode))
+
+    def set_graph(self, enable: bool = True):
+        _check(lib().msosm_multi_set_graph(self.h, int(enable)))
 
     def push(self, x) -> None:
         if isinstance(x, int):

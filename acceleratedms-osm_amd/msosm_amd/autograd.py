@@ -1,0 +1,88 @@
+"""DM gradients of folded profiles through torch.autograd, and gradient DM refinement.
+
+``FoldedProfile`` maps trial DMs to folded profiles of a device-resident baseband record (MS-OSM
+coherent dedispersion + fold, msosm_gpu_batch.cpp:178-236 and fold_gpu.cpp:48-60, on the GPU). Its
+backward recomputes the dedispersed series: each block is dedispersed with H and with dH/dDM and
+2 Re(conj(y) dy/dDM) is folded with the forward bins, which gives d profile / d DM; nothing from the
+forward pass besides the DMs is kept. The chain rule with the incoming profile gradient gives the
+DM gradient, so any torch loss on the profiles works (``snr_loss`` is the one used for refinement).
+
+The segmentation (M, delay blocks) is frozen at the engine's starting DMs; only the chirp follows
+the trial DMs, so the profile is a smooth function of DM.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import lib as _load_library
+
+# torch bundles its own HIP runtime; it must start before this library's (which starts on its first
+# HIP call), or torch finds no GPU. Starting torch's here is a no-op where it already runs.
+if torch.cuda.is_available():
+    torch.cuda.init()
+_load_library()  # the C ABI library must be present (no CPU fallback)
+from . import _core  # noqa: E402  (pybind11 module, linked against the library)
+
+ProfileEngine = _core.ProfileEngine
+
+
+class FoldedProfile(torch.autograd.Function):
+    """profile[D][nbins] (float64, CPU) = fold(|MS-OSM(record; DM_d)|^2); differentiable in dm."""
+
+    @staticmethod
+    def forward(ctx, dm: torch.Tensor, engine, record: torch.Tensor, n_samples: int):
+        engine.set_dms(dm.detach().cpu().numpy().astype(np.float32))
+        prof, _, _ = engine.run(record.data_ptr(), int(n_samples), False)
+        ctx.engine, ctx.record, ctx.n = engine, record, int(n_samples)
+        ctx.save_for_backward(dm)
+        return torch.from_numpy(prof)
+
+    @staticmethod
+    def backward(ctx, grad_profile: torch.Tensor):
+        (dm,) = ctx.saved_tensors
+        engine = ctx.engine
+        engine.set_dms(dm.detach().cpu().numpy().astype(np.float32))
+        _, dprof, _ = engine.run(ctx.record.data_ptr(), ctx.n, True)
+        g = (grad_profile.detach().to(torch.float64).cpu() * torch.from_numpy(dprof)).sum(dim=1)
+        return g.to(dm.dtype), None, None, None
+
+
+def folded_profile(dm: torch.Tensor, engine, record: torch.Tensor, n_samples: int | None = None) -> torch.Tensor:
+    n = int(record.shape[0]) if n_samples is None else int(n_samples)
+    return FoldedProfile.apply(dm, engine, record, n)
+
+
+def snr_loss(profile: torch.Tensor, noise_var: float) -> torch.Tensor:
+    """Per-DM loss -sum_b (p_b - mean(p))^2 / noise_var (lower is better)."""
+    c = profile - profile.mean(dim=1, keepdim=True)
+    return -(c * c).sum(dim=1) / noise_var
+
+
+def profile_noise_var(record: torch.Tensor, n_samples: int, nbins: int) -> float:
+    """Variance of one profile bin under noise alone: var(|x|^2) * nbins / n (fixed per record)."""
+    x = record[:n_samples]
+    if x.dtype == torch.int8:  # interleaved I/Q bytes, shape (n, 2)
+        p = x.to(torch.float32).pow(2).sum(dim=-1)
+    else:
+        p = x.abs().pow(2).to(torch.float32)
+    return float(p.to(torch.float64).var()) * nbins / n_samples
+
+
+def refine_dm(engine, record: torch.Tensor, dm_start, noise_var: float, n_samples: int | None = None,
+              steps: int = 40, step0: float = 0.02, step_min: float = 1e-6, step_max: float = 0.2):
+    """Gradient DM refinement of every trial at once with sign-based steps (torch.optim.Rprop).
+
+    Each step runs one forward and one backward (the derivative pass). Returns
+    (refined DMs, final losses, number of steps taken)."""
+    dm = torch.tensor(np.asarray(dm_start, dtype=np.float64), requires_grad=True)
+    opt = torch.optim.Rprop([dm], lr=step0, etas=(0.5, 1.2), step_sizes=(step_min, step_max))
+    loss = None
+    for it in range(steps):
+        opt.zero_grad()
+        loss = snr_loss(folded_profile(dm, engine, record, n_samples), noise_var)
+        loss.sum().backward()
+        opt.step()
+    with torch.no_grad():
+        final = snr_loss(folded_profile(dm.detach(), engine, record, n_samples), noise_var)
+    return dm.detach().numpy(), final.numpy(), steps

@@ -23,6 +23,12 @@ for p in (ROOT, os.path.join(ROOT, "acceleratedms-osm_amd")):
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 
+# Uneven grid (--workload uneven): complex64 baseband, 2^26 samples at 32 MS/s, band 784-816 MHz,
+# 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
+# that DM, 2^14..2^18 here), complex64 series out; one launch over all lengths vs one per DM.
+UNEVEN = dict(workload="msosm_uneven_c64", samples=1 << 26, fs_hz=32e6, fc_hz=800e6, bw_hz=32e6, num_dms=1024,
+              dm_lo=0.0, dm_hi=1000.0, push_blocks=1, output="complex64 series", input="complex64, normal")
+
 WORKLOAD = dict(workload="msosm_dm_grid_int8", samples=1 << 27, fs_hz=64e6, fc_hz=1250e6, bw_hz=64e6,
                 num_dms=512, dm_lo=0.0, dm_hi=100.0, fft_len=1 << 16, batch=32, nbins=1024, period_s=5.0e-3,
                 input="int8 I/Q, uniform in [-127,127]")
@@ -62,6 +68,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
+    ap.add_argument("--workload", choices=["dm_grid", "uneven"], default="dm_grid")
+    ap.add_argument("--plain-pushes", type=int, default=8,
+                    help="uneven workload: pushes timed in the one-launch-per-DM form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -79,6 +88,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               f"path (DM sharding, no exchange) is not built.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "uneven":
+        bench_uneven(args)
+        return
 
     import numpy as np
     import torch
@@ -164,6 +177,71 @@ def main():
     print(json.dumps(out), flush=True)
     plan.close()
     fold.close()
+
+
+def bench_uneven(args):
+    """Uneven DM grid: the one-launch form over all length classes against one launch per DM."""
+    import numpy as np
+    import torch
+
+    import msosm_amd as ms
+
+    W = UNEVEN
+    f0 = W["fc_hz"] - W["bw_hz"] / 2
+    dms = np.linspace(W["dm_lo"], W["dm_hi"], W["num_dms"]).astype(np.float32)
+    t = time.perf_counter()
+    design = ms.UnevenDesign(W["bw_hz"], f0, dms)
+    multi = ms.Multi(design, push_blocks=W["push_blocks"], in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
+    lengths = {int(2 * m): int((design.M == m).sum()) for m in np.unique(design.M)}
+    log(f"design + plans: {multi.num_classes} length classes {lengths} ({time.perf_counter() - t:.1f} s)")
+    N, D, per = W["samples"], W["num_dms"], multi.push_samples
+    pushes = N // per
+    if args.pushes:
+        pushes = min(pushes, args.pushes)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev).manual_seed(4321)
+    data = torch.randn(N, dtype=torch.complex64, device=dev, generator=gen)
+    torch.cuda.synchronize()
+    base = data.data_ptr()
+
+    def run(n, first=0):
+        for c in range(n):
+            multi.push(base + ((first + c) % (N // per)) * per * 8)
+        multi.synchronize()
+
+    res = {}
+    for name, mode, n in (("one_launch", ms.LAUNCH_PERSISTENT, pushes),
+                          ("per_dm", ms.LAUNCH_PER_DM, min(args.plain_pushes, pushes))):
+        multi.set_launch_mode(mode)
+        run(2)
+        multi.profile(True)
+        t0 = time.perf_counter()
+        steps = args.steps if name == "one_launch" else 1
+        for _ in range(steps):
+            run(n)
+        t1 = time.perf_counter()
+        kms, kn = multi.profile_read(1)
+        multi.profile(False)
+        res[name] = dict(rate=D * n * per * steps / (t1 - t0), ms_per_push=(t1 - t0) * 1e3 / (n * steps),
+                         kernel_ms=kms, launches=kn, pushes=n * steps)
+        log(f"{name}: {res[name]['rate']:.3e} DM-trial*samples/s, {res[name]['ms_per_push']:.3f} ms/push, "
+            f"{kn} dedispersion launches, {kms:.1f} ms in them")
+    one = res["one_launch"]
+    alg = D * per * 24.0  # 16 B gathered spectrum + 8 B complex64 output per DM-sample
+    avg_s = one["kernel_ms"] / one["launches"] / 1e3
+    roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": alg / avg_s / HBM_PEAK, "traffic": None, "kernel": "inv_multi_kernel",
+            "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "alg_bytes_per_launch": alg}
+    out = {
+        "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 2, "ms_per_step": one["ms_per_push"] * pushes, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": dict(W, length_classes=lengths), "roofline": roof, "cpu_baseline": None,
+        "per_dm_launch_form": {"value": res["per_dm"]["rate"], "pushes": res["per_dm"]["pushes"],
+                               "speedup_of_one_launch": one["rate"] / res["per_dm"]["rate"]},
+    }
+    print(json.dumps(out), flush=True)
+    multi.close()
 
 
 if __name__ == "__main__":

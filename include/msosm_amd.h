@@ -84,6 +84,9 @@ int msosm_design_transfer_function(float bw, float f0, float dm, int M, int phas
 /* Same for num_dms DMs into H[num_dms][2M] using up to nthreads host threads (0 = all cores). */
 int msosm_design_transfer_functions(float bw, float f0, const float *dms, int num_dms, int M, int phase_mode,
                                     float *H, int nthreads);
+/* dH/dDM[2M] complex: the same design map applied to dc/dDM = -i (phi/DM) c, the analytic derivative of the
+ * chirp of msosm_prepare.cpp:122-138 (no reference counterpart; it feeds the DM-gradient pass). */
+int msosm_design_transfer_function_ddm(float bw, float f0, float dm, int M, int phase_mode, float *dH);
 
 /* ------------------------------------------------------------------------------------------------
  * Device plan: the MSOSM_GPU_BATCH device pipeline (msosm_gpu_batch.cpp:12-151,178-245).
@@ -143,8 +146,9 @@ int msosm_plan_get_info(msosm_plan *plan, msosm_plan_info *info);
 /* How push launches the dedispersion kernel */
 enum {
     MSOSM_LAUNCH_PERSISTENT = 0, /* one persistent launch over every (DM, block) unit (default) */
-    MSOSM_LAUNCH_PER_DM = 1      /* one launch per DM, as the reference's per-DM inverse loop
+    MSOSM_LAUNCH_PER_DM = 1,     /* one launch per DM, as the reference's per-DM inverse loop
                                     (msosm_gpu_batch.cpp:196-199); kept as the comparison form */
+    MSOSM_LAUNCH_PER_CLASS = 2   /* uneven grids only: one persistent launch per FFT length */
 };
 int msosm_plan_set_launch_mode(msosm_plan *plan, int mode);
 /* HIP-event timing of the plan's kernels. kernel: 0 = forward FFT, 1 = dedispersion */
@@ -175,6 +179,15 @@ int msosm_fold_get_nbins(msosm_fold *fold, unsigned long *nbins);
  * kernel (size must equal batch*M, num_series == num_dms, nbins <= 2048). NULL detaches. */
 int msosm_plan_attach_fold(msosm_plan *plan, msosm_fold *fold);
 
+/* DM-derivative pass (no reference counterpart; for gradient DM refinement). dH[2M] from
+ * msosm_design_transfer_function_ddm for DM i. */
+int msosm_plan_set_dm_tangent(msosm_plan *plan, int i, const float *dH);
+/* A push that dedisperses with H (folding |y|^2 into the attached fold, if any) and with dH/dDM,
+ * folding d|y|^2/dDM = 2 Re(conj(y) dy/dDM) into tangent_fold (same bins, size batch*M, one series
+ * per DM; samples with |y|^2 == 0 are skipped as in the primal fold). tangent_fold's profile is then
+ * d profile / d DM. No series outputs are written by this push. */
+int msosm_plan_push_tangent(msosm_plan *plan, const void *input, int input_mode, msosm_fold *tangent_fold);
+
 /* ------------------------------------------------------------------------------------------------
  * Uneven DM grids (no reference counterpart: the reference uses one M for every DM,
  * msosm_prepare.cpp:67). Each DM keeps its own M; DMs of equal M run as one plan ("length class");
@@ -202,7 +215,15 @@ int msosm_multi_destroy(msosm_multi *multi);
 int msosm_multi_get_info(msosm_multi *multi, int dm, int *M, int *num_classes, long long *push_samples);
 /* delay_points and H hold 2*M[dm] entries */
 int msosm_multi_set_dm(msosm_multi *multi, int dm, const int *delay_points, const float *H);
+/* MSOSM_LAUNCH_PERSISTENT (default): one persistent launch over the units of every length class
+ * (lengths 2^14..2^21, at most 8 classes, rings below 4 GiB each, fold bins within the LDS left by
+ * the largest tile set; otherwise one persistent launch per class). MSOSM_LAUNCH_PER_CLASS: one
+ * persistent launch per class. MSOSM_LAUNCH_PER_DM: one launch per DM. */
 int msosm_multi_set_launch_mode(msosm_multi *multi, int mode);
+/* enable != 0: in the one-launch form, run each push as one hipGraph (counter fill, per-class forward
+ * transforms, history and fold bins, then the multi-length dedispersion), built on the first push and
+ * re-parameterised on later ones. Profiling (kernel 1) then times the whole push graph. */
+int msosm_multi_set_graph(msosm_multi *multi, int enable);
 int msosm_multi_push(msosm_multi *multi, const void *input, int input_mode);
 /* complex64 series of one DM for the last push (push_samples values) */
 int msosm_multi_output_c64(msosm_multi *multi, int dm, void **dev_ptr);

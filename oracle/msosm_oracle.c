@@ -222,13 +222,22 @@ int orc_build_segments(const int *delay, int fft_len, int max_len, int *seg, int
     return count + 1;
 }
 
+static int transfer_impl(float bw, float f0, float dm, int M, int phase_fp32, int deriv, double *H);
+
 int orc_transfer_function(float bw, float f0, float dm, int M, int phase_fp32, double *H)
 {
+    return transfer_impl(bw, f0, dm, M, phase_fp32, 0, H);
+}
+
+int orc_transfer_function_ddm(float bw, float f0, float dm, int M, int phase_fp32, double *dH)
+{
+    return transfer_impl(bw, f0, dm, M, phase_fp32, 1, dH);
+}
+
+/* phase of design bin j at a given DM (msosm_prepare.cpp:122-134; legacy :179-192) */
+static double chirp_phase(float bw, float f0, float dm, int M, int j, int phase_fp32)
+{
     /* msosm_prepare.cpp:107-164 (FP64 phase) and the legacy FP32 phase of :166-213 */
-    if (M <= 0 || (M & (M - 1)) != 0 || !H) return -1;
-    const long L = 2L * M;
-    double *h = (double *)calloc((size_t)(2 * M), sizeof(double));
-    if (!h) return -1;
     const double pi_d = 3.14159265358979323846;
     const double kdm_d = 4.15e15;
     const double fs_d = (double)bw;
@@ -236,7 +245,7 @@ int orc_transfer_function(float bw, float f0, float dm, int M, int phase_fp32, d
     const double f0_d = (double)f0;
     const double w0_d = 2.0 * pi_d * f0_d;
     const double step = 2.0 * pi_d * fs_d / (double)M;
-    for (int j = 0; j < M; j++) {
+    {
         double phase;
         if (!phase_fp32) {
             double w = step * (double)j;
@@ -253,9 +262,30 @@ int orc_transfer_function(float bw, float f0, float dm, int M, int phase_fp32, d
             double num = 4 * ref_pi * ref_pi * (double)kdm * (double)dm * (double)wf * (double)wf;
             phase = num / (double)(wf + w0) / (double)w0 / (double)w0;
         }
+        return phase;
+    }
+}
+
+/* H = DFT_2M([IDFT_M(c)/M, 0]) with c_n = e^{-i phase}; with deriv, c is replaced by
+ * dc/dDM = -i (phase at DM 1) c (the phase is proportional to DM), which gives dH/dDM */
+static int transfer_impl(float bw, float f0, float dm, int M, int phase_fp32, int deriv, double *H)
+{
+    if (M <= 0 || (M & (M - 1)) != 0 || !H) return -1;
+    const long L = 2L * M;
+    double *h = (double *)calloc((size_t)(2 * M), sizeof(double));
+    if (!h) return -1;
+    for (int j = 0; j < M; j++) {
+        const double phase = chirp_phase(bw, f0, dm, M, j, phase_fp32);
+        const double cr = cos(phase), ci = -sin(phase);
         int n = (j < M / 2) ? (M / 2 + j) : (j - M / 2);
-        h[2 * n] = cos(phase);
-        h[2 * n + 1] = -sin(phase);
+        if (!deriv) {
+            h[2 * n] = cr;
+            h[2 * n + 1] = ci;
+        } else {
+            const double u = chirp_phase(bw, f0, 1.0f, M, j, phase_fp32);
+            h[2 * n] = u * ci; /* -i u (cr + i ci) = u ci - i u cr */
+            h[2 * n + 1] = -u * cr;
+        }
     }
     /* h = IDFT_M(c) / M (e^{+i}); H = DFT_2M([h, 0]) (e^{-i}) */
     if (orc_fft(h, M, +1) != 0) {

@@ -42,6 +42,9 @@ int orc_build_segments(const int *delay, int fft_len, int max_len, int *seg, int
  * (FP64 phase, FP64 FFTs). H is 2M complex values, interleaved re/im doubles.
  * phase_fp32 != 0 selects the legacy FP32-phase form of msosm_prepare.cpp:166-213. */
 int orc_transfer_function(float bw, float f0, float dm, int M, int phase_fp32, double *H);
+/* dH/dDM of the same design (no reference counterpart): the design map is linear in the chirp, and
+ * the phase is proportional to DM, so dc/dDM = -i (phase at DM 1) c. */
+int orc_transfer_function_ddm(float bw, float f0, float dm, int M, int phase_fp32, double *dH);
 
 /* ---- MS-OSM / OSM (FP64) ----
  * A6-A7 of the reference path: windows x[(g-1)M,(g+1)M) with x[<0]=0 (msosm_gpu_batch.cpp:47-52,

@@ -38,6 +38,7 @@ def lib():
         L.orc_segmentation.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, ip]
         L.orc_build_segments.argtypes = [ip, C.c_int, C.c_int, ip, C.c_int]
         L.orc_transfer_function.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, dp]
+        L.orc_transfer_function_ddm.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, dp]
         L.orc_msosm.argtypes = [dp, C.c_long, C.c_int, C.c_int, ip, dp, dp]
         L.orc_fold.argtypes = [dp, C.c_long, C.c_ulong, C.c_double, C.c_double, C.c_double, dp,
                                C.POINTER(C.c_ulonglong), lp]
@@ -112,6 +113,13 @@ def transfer_function(bw: float, f0: float, dm: float, M: int, phase_fp32: bool 
     return H
 
 
+def transfer_function_ddm(bw: float, f0: float, dm: float, M: int, phase_fp32: bool = False) -> np.ndarray:
+    dH = np.empty(2 * M, dtype=np.complex128)
+    if lib().orc_transfer_function_ddm(bw, f0, dm, M, int(phase_fp32), _dp(dH.view(np.float64))) != 0:
+        raise ValueError("transfer_function_ddm failed")
+    return dH
+
+
 def msosm(x: np.ndarray, M: int, delays: np.ndarray, H: np.ndarray) -> np.ndarray:
     """FP64 MS-OSM over the whole record; delays [D][2M], H [D][2M] -> out [D][N]."""
     xx = np.ascontiguousarray(x, dtype=np.complex128)
@@ -140,6 +148,34 @@ def fold(intensity: np.ndarray, nbins: int, t_start: float, dt: float, period: f
                    counts.ctypes.data_as(C.POINTER(C.c_ulonglong)),
                    bins.ctypes.data_as(C.POINTER(C.c_long)) if want_bins else None)
     return (sums, counts, bins) if want_bins else (sums, counts)
+
+
+def profile_tangent(y: np.ndarray, dy: np.ndarray, nbins: int, t_start: float, dt: float, period: float,
+                    call_len: int | None = None):
+    """Folded profile p = sum(|y|^2)/count per bin and its derivative dp = sum(2 Re(conj(y) dy))/count.
+
+    y is one dedispersed series and dy its derivative with respect to DM (the MS-OSM output is linear
+    in H, so dy is the MS-OSM of the same input with dH/dDM). Samples with |y|^2 == 0 are skipped in
+    both, as foldDataPhase_kernel (gpu/gpu_kernel.cu:462-496) skips them; counts do not depend on DM.
+    With call_len, the series is folded in calls of call_len samples whose clock advances by
+    call_len * dt per call, as Fold_GPU::fold_data_phase does (folding/fold_gpu.cpp:48-54)."""
+    I = np.abs(y) ** 2
+    J = 2.0 * (y.real * dy.real + y.imag * dy.imag)
+    step = I.size if call_len is None else call_len
+    sums = np.zeros(nbins, dtype=np.float64)
+    counts = np.zeros(nbins, dtype=np.uint64)
+    dsums = np.zeros(nbins, dtype=np.float64)
+    t0 = t_start
+    for s0 in range(0, I.size, step):
+        seg = slice(s0, min(s0 + step, I.size))
+        _, _, bins = fold(I[seg], nbins, t0, dt, period, sums, counts, want_bins=True)
+        keep = I[seg] != 0.0
+        np.add.at(dsums, bins[keep], J[seg][keep])
+        t0 += float(step) * dt
+    with np.errstate(invalid="ignore", divide="ignore"):
+        p = np.where(counts > 0, sums / np.maximum(counts, 1), 0.0)
+        dp = np.where(counts > 0, dsums / np.maximum(counts, 1), 0.0)
+    return p, dp, counts
 
 
 def fold_bin(i: int, nbins: int, t_start: float, dt: float, period: float) -> int:

@@ -88,3 +88,56 @@ def test_per_dm_launch_form_is_bitwise_identical():
         res.append(np.concatenate(o, axis=1))
         plan.close()
     assert np.array_equal(res[0], res[1])
+
+
+def test_uneven_launch_forms_agree():
+    # one launch over all lengths, one launch per length and one launch per DM run the same unit
+    # code: the series are bit-identical and the fold counts equal
+    design = ms.UnevenDesign(BW, F0, [0.0, 15.0, 120.0, 400.0, 900.0, 60.0])
+    calls, nbins, period = 2, 64, 0.0011
+    res = []
+    for mode in (ms.LAUNCH_PERSISTENT, ms.LAUNCH_PER_CLASS, ms.LAUNCH_PER_DM):
+        multi = ms.Multi(design, push_blocks=2)
+        multi.set_launch_mode(mode)
+        multi.fold(period, BW, nbins)
+        n = multi.push_samples * calls
+        raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 400.0, period, seed=34)
+        outs = []
+        for c in range(calls):
+            multi.push(raw[c * multi.push_samples:(c + 1) * multi.push_samples])
+            outs.append(np.stack([multi.output(i) for i in range(multi.D)]))
+        _, sums, cnts = multi.fold_read()
+        print(f"mode {mode}: classes {multi.num_classes}", flush=True)
+        res.append((np.concatenate(outs, axis=1), sums, cnts))
+        multi.close()
+    for r in res[1:]:
+        assert np.array_equal(res[0][0], r[0])
+        assert np.array_equal(res[0][2], r[2])
+        assert np.allclose(res[0][1], r[1], rtol=1e-12, atol=0)
+
+
+def test_graph_push_replays_match_eager():
+    # the push as one hipGraph (built on the first push, re-parameterised after): every replay
+    # equals the eager one-launch push bit for bit, with host copies and syncs between replays
+    design = ms.UnevenDesign(BW, F0, [0.0, 15.0, 120.0, 400.0, 900.0])
+    calls, nbins, period = 4, 64, 0.0011
+    res = []
+    for graph in (False, True):
+        multi = ms.Multi(design, push_blocks=2)
+        multi.fold(period, BW, nbins)
+        multi.set_graph(graph)
+        n = multi.push_samples * calls
+        raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 400.0, period, seed=35)
+        outs = []
+        for c in range(calls):
+            multi.push(raw[c * multi.push_samples:(c + 1) * multi.push_samples])
+            outs.append(np.stack([multi.output(i) for i in range(multi.D)]))
+            multi.synchronize()
+        _, sums, cnts = multi.fold_read()
+        print(f"graph={graph}: pushed {calls}", flush=True)
+        res.append((outs, sums, cnts))
+        multi.close()
+    for c in range(calls):
+        assert np.array_equal(res[0][0][c], res[1][0][c]), f"push {c}"
+    assert np.array_equal(res[0][2], res[1][2])
+    assert np.allclose(res[0][1], res[1][1], rtol=1e-12, atol=0)
