@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "msosm_amd.h"
@@ -77,7 +78,7 @@ class ProfileEngine {
         check(msosm_fold_create(period, bw, size, nbins, D_, device, &fold_), "fold_create");
         check(msosm_fold_create(period, bw, size, nbins, D_, device, &tfold_), "fold_create");
         check(msosm_plan_attach_fold(plan_, fold_), "plan_attach_fold");
-        set_dms(start_dms);
+        set_dms(start_dms, false);
     }
 
     ~ProfileEngine()
@@ -87,22 +88,37 @@ class ProfileEngine {
         if (tfold_) msosm_fold_destroy(tfold_);
     }
 
-    // new trial DMs (the segmentation stays that of the starting DMs)
-    void set_dms(py::array_t<float, py::array::c_style | py::array::forcecast> dms)
+    // new trial DMs (the segmentation stays that of the starting DMs); with_tangent also designs and
+    // uploads dH/dDM, which the tangent run needs
+    void set_dms(py::array_t<float, py::array::c_style | py::array::forcecast> dms, bool with_tangent)
     {
         if (dms.size() != D_) throw std::invalid_argument("set_dms: expected " + std::to_string(D_) + " DMs");
         cur_.assign(dms.data(), dms.data() + D_);
-        std::vector<float> H((size_t)D_ * 4 * M_), dH((size_t)4 * M_);
+        std::vector<float> H((size_t)D_ * 4 * M_);
         {
             py::gil_scoped_release nogil;
             check(msosm_design_transfer_functions(bw_, f0_, cur_.data(), D_, M_, phase_mode_, H.data(), 0),
                   "design_transfer_functions");
-            for (int i = 0; i < D_; ++i) {
+            for (int i = 0; i < D_; ++i)
                 check(msosm_plan_set_dm(plan_, i, delays_.data() + (size_t)i * 2 * M_, H.data() + (size_t)i * 4 * M_),
                       "plan_set_dm");
-                check(msosm_design_transfer_function_ddm(bw_, f0_, cur_[i], M_, phase_mode_, dH.data()),
-                      "design_transfer_function_ddm");
-                check(msosm_plan_set_dm_tangent(plan_, i, dH.data()), "plan_set_dm_tangent");
+            tangent_ready_ = false;
+            if (with_tangent) {
+                // dH/dDM per DM on all host cores (H's buffer is reused)
+                const int nt = std::max(1, std::min<int>(D_, (int)std::thread::hardware_concurrency()));
+                std::vector<int> rc(nt, MSOSM_OK);
+                std::vector<std::thread> pool;
+                for (int t = 0; t < nt; ++t)
+                    pool.emplace_back([&, t] {
+                        for (int i = t; i < D_ && rc[t] == MSOSM_OK; i += nt)
+                            rc[t] = msosm_design_transfer_function_ddm(bw_, f0_, cur_[i], M_, phase_mode_,
+                                                                       H.data() + (size_t)i * 4 * M_);
+                    });
+                for (auto &th : pool) th.join();
+                for (int r : rc) check(r, "design_transfer_function_ddm");
+                for (int i = 0; i < D_; ++i)
+                    check(msosm_plan_set_dm_tangent(plan_, i, H.data() + (size_t)i * 4 * M_), "plan_set_dm_tangent");
+                tangent_ready_ = true;
             }
         }
     }
@@ -114,6 +130,7 @@ class ProfileEngine {
         const long long per = (long long)B_ * M_;
         if (n_samples <= 0 || n_samples % per)
             throw std::invalid_argument("run: n_samples must be a positive multiple of batch*M = " + std::to_string(per));
+        if (tangent && !tangent_ready_) throw std::runtime_error("run: tangent needs set_dms(..., with_tangent=True)");
         const size_t bytes_per = in_format_ == MSOSM_IN_U16 ? 4 : in_format_ == MSOSM_IN_I8 ? 2 : 8;
         const size_t nb = (size_t)D_ * nbins_;
         std::vector<double> sums(nb), tsums(nb);
@@ -167,6 +184,7 @@ class ProfileEngine {
     double period_;
     int in_format_, phase_mode_;
     int D_ = 0, M_ = 0, dc_ = 0;
+    bool tangent_ready_ = false;
     std::vector<float> start_, cur_;
     std::vector<int> delays_;
     msosm_plan *plan_ = nullptr;
@@ -184,7 +202,7 @@ PYBIND11_MODULE(_core, m)
              py::arg("bw"), py::arg("f0"), py::arg("start_dms"), py::arg("batch"), py::arg("nbins"),
              py::arg("period"), py::arg("in_format") = (int)MSOSM_IN_C64, py::arg("fftpoint") = 0,
              py::arg("phase_mode") = (int)MSOSM_PHASE_FP64, py::arg("device") = 0)
-        .def("set_dms", &ProfileEngine::set_dms, py::arg("dms"))
+        .def("set_dms", &ProfileEngine::set_dms, py::arg("dms"), py::arg("with_tangent") = false)
         .def("run", &ProfileEngine::run, py::arg("input"), py::arg("n_samples"), py::arg("tangent") = false)
         .def_property_readonly("M", &ProfileEngine::M)
         .def_property_readonly("delaycount", &ProfileEngine::delaycount)

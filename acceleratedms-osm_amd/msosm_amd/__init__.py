@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(_HERE, "_lib", "libmsosm_amd.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ALLOC = range(6)
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ALLOC, ERR_IO = range(7)
+SAVE_URING, SAVE_DIRECT, SAVE_PINNED = 1, 2, 4
 IN_U16, IN_I8, IN_C64 = 0, 1, 2
 OUT_NONE, OUT_C64, OUT_U16, OUT_F16 = 0, 1, 2, 4
 PHASE_FP64, PHASE_FP32 = 0, 1
@@ -39,6 +40,8 @@ EXPORTS = [
     "msosm_multi_profile", "msosm_multi_profile_read",
     "msosm_design_transfer_function_ddm", "msosm_plan_set_dm_tangent", "msosm_plan_push_tangent",
     "msosm_multi_set_graph",
+    "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
+    "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 
@@ -135,6 +138,15 @@ def lib():
         "msosm_plan_set_dm_tangent": ([vp, C.c_int, fp], C.c_int),
         "msosm_plan_push_tangent": ([vp, vp, C.c_int, vp], C.c_int),
         "msosm_multi_set_graph": ([vp, C.c_int], C.c_int),
+        "msosm_saver_create": ([C.POINTER(C.c_char_p), C.c_int, C.c_size_t, C.c_int, C.c_ulong, C.c_int,
+                                C.POINTER(vp)], C.c_int),
+        "msosm_saver_start": ([vp], C.c_int),
+        "msosm_saver_acquire": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_saver_commit_stream": ([vp, vp], C.c_int),
+        "msosm_saver_commit_ready": ([vp], C.c_int),
+        "msosm_saver_join": ([vp], C.c_int),
+        "msosm_saver_destroy": ([vp], C.c_int),
+        "msosm_saver_file_name": ([C.c_char_p, C.c_char_p, fp, C.c_int, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -510,6 +522,60 @@ class Multi:
     def close(self):
         if self.h:
             lib().msosm_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def saver_file_name(directory: str, prefix: str, dms, i: int) -> str:
+    """<dir>/<prefix>_DM<value>.dat with the reference's DM precision rule."""
+    d = np.ascontiguousarray(dms, dtype=np.float32)
+    buf = C.create_string_buffer(4096)
+    _check(lib().msosm_saver_file_name(directory.encode(), prefix.encode(), _fp(d), d.size, i, buf, 4096))
+    return buf.value.decode()
+
+
+class Saver:
+    """Output pipeline: slots of num_files * block_bytes, written to one file per series (C ABI
+    msosm_saver_*; MSOSM_DataSave_Uring's slot protocol)."""
+
+    def __init__(self, paths, block_bytes: int, slot_count: int, save_count: int, flags: int = SAVE_URING):
+        arr = (C.c_char_p * len(paths))(*[p.encode() for p in paths])
+        h = C.c_void_p()
+        _check(lib().msosm_saver_create(arr, len(paths), block_bytes, slot_count, save_count, flags, C.byref(h)))
+        self.h, self.n, self.block = h, len(paths), block_bytes
+
+    def start(self):
+        _check(lib().msosm_saver_start(self.h))
+
+    def push_host(self, data: np.ndarray):
+        """Copy one push (num_files * block_bytes bytes) into the next slot and commit it."""
+        b = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        if b.size != self.n * self.block:
+            raise ValueError("push_host: wrong size")
+        p = C.c_void_p()
+        _check(lib().msosm_saver_acquire(self.h, C.byref(p)))
+        C.memmove(p, b.ctypes.data, b.size)
+        _check(lib().msosm_saver_commit_ready(self.h))
+
+    def acquire(self) -> int:
+        p = C.c_void_p()
+        _check(lib().msosm_saver_acquire(self.h, C.byref(p)))
+        return p.value
+
+    def commit_stream(self, stream: int | None):
+        _check(lib().msosm_saver_commit_stream(self.h, C.c_void_p(stream or 0)))
+
+    def join(self):
+        _check(lib().msosm_saver_join(self.h))
+
+    def close(self):
+        if self.h:
+            lib().msosm_saver_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -42,7 +42,7 @@ class FoldedProfile(torch.autograd.Function):
     def backward(ctx, grad_profile: torch.Tensor):
         (dm,) = ctx.saved_tensors
         engine = ctx.engine
-        engine.set_dms(dm.detach().cpu().numpy().astype(np.float32))
+        engine.set_dms(dm.detach().cpu().numpy().astype(np.float32), True)
         _, dprof, _ = engine.run(ctx.record.data_ptr(), ctx.n, True)
         g = (grad_profile.detach().to(torch.float64).cpu() * torch.from_numpy(dprof)).sum(dim=1)
         return g.to(dm.dtype), None, None, None

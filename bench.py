@@ -29,6 +29,13 @@ HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 UNEVEN = dict(workload="msosm_uneven_c64", samples=1 << 26, fs_hz=32e6, fc_hz=800e6, bw_hz=32e6, num_dms=1024,
               dm_lo=0.0, dm_hi=1000.0, push_blocks=1, output="complex64 series", input="complex64, normal")
 
+# DM refinement (--workload refine): complex64 baseband, 2^24 samples at 16 MS/s, band 1392-1408 MHz,
+# a dispersed pulsar (DM 123.456, P = 33 ms, Gaussian pulse of 3 % FWHM, amplitude 0.5, AWGN sigma 1),
+# 512-bin folds; 64 start DMs within +-0.5 refined by DM gradients vs a 1e-4 grid over +-0.5.
+REFINE = dict(workload="msosm_dm_refine", samples=1 << 24, fs_hz=16e6, fc_hz=1400e6, bw_hz=16e6, true_dm=123.456,
+              period_s=33.0e-3, duty=0.03, amp=0.5, sigma=1.0, nbins=512, num_starts=64, start_halfwidth=0.5,
+              grid_step=1e-4, batch=32)
+
 WORKLOAD = dict(workload="msosm_dm_grid_int8", samples=1 << 27, fs_hz=64e6, fc_hz=1250e6, bw_hz=64e6,
                 num_dms=512, dm_lo=0.0, dm_hi=100.0, fft_len=1 << 16, batch=32, nbins=1024, period_s=5.0e-3,
                 input="int8 I/Q, uniform in [-127,127]")
@@ -68,7 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
-    ap.add_argument("--workload", choices=["dm_grid", "uneven"], default="dm_grid")
+    ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
+    ap.add_argument("--refine-steps", type=int, default=40)
     ap.add_argument("--plain-pushes", type=int, default=8,
                     help="uneven workload: pushes timed in the one-launch-per-DM form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -91,6 +99,9 @@ def main():
 
     if args.workload == "uneven":
         bench_uneven(args)
+        return
+    if args.workload == "refine":
+        bench_refine(args)
         return
 
     import numpy as np
@@ -242,6 +253,77 @@ def bench_uneven(args):
     }
     print(json.dumps(out), flush=True)
     multi.close()
+
+
+def synth_pulsar(n, fs, f0, dm, period, duty, amp, sigma, seed, device):
+    """Dispersed pulsar baseband on the GPU: complex unit-variance noise modulated by a Gaussian pulse
+    (FWHM = duty * period, peak amplitude amp), dispersed with the direct chirp e^{+i phi} over the
+    whole record (FP64), plus complex AWGN of standard deviation sigma."""
+    import math
+
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    t = torch.arange(n, device=device, dtype=torch.float64) / fs
+    ph = torch.remainder(t / period, 1.0) - 0.5
+    sig = duty / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    a = amp * torch.exp(-0.5 * (ph / sig) ** 2)
+    x = a * torch.randn(n, dtype=torch.complex128, device=device, generator=g)
+    k = torch.arange(n, device=device, dtype=torch.float64)
+    f = torch.where(k < n // 2, fs / 2 + k * fs / n, k * fs / n - fs / 2)
+    w, w0 = 2 * math.pi * f, 2 * math.pi * f0
+    phase = 4 * math.pi ** 2 * 4.15e15 * dm * w * w / ((w + w0) * w0 * w0)
+    x = torch.fft.ifft(torch.fft.fft(x) * torch.polar(torch.ones_like(phase), phase))
+    x = x + sigma * torch.randn(n, dtype=torch.complex128, device=device, generator=g)
+    return x.to(torch.complex64)
+
+
+def bench_refine(args):
+    """Gradient DM refinement (forward fold + DM-derivative pass per step) against a fine DM grid."""
+    import numpy as np
+    import torch
+
+    from msosm_amd import autograd as ag
+
+    W = REFINE
+    f0 = W["fc_hz"] - W["bw_hz"] / 2
+    dev = torch.device("cuda:0")
+    N = W["samples"]
+    rec = synth_pulsar(N, W["fs_hz"], f0, W["true_dm"], W["period_s"], W["duty"], W["amp"], W["sigma"], 2024, dev)
+    torch.cuda.synchronize()
+    noise_var = ag.profile_noise_var(rec, N, W["nbins"])
+    rng = np.random.default_rng(20261203)
+    starts = W["true_dm"] + rng.uniform(-W["start_halfwidth"], W["start_halfwidth"], W["num_starts"])
+    eng = ag.ProfileEngine(W["bw_hz"], f0, starts.astype(np.float32), W["batch"], W["nbins"], W["period_s"])
+    log(f"refine: M={eng.M} delaycount={eng.delaycount} noise_var={noise_var:.3e}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dm_ref, loss_ref, steps = ag.refine_dm(eng, rec, starts, noise_var, steps=args.refine_steps)
+    t_ref = time.perf_counter() - t0
+    best = int(np.argmin(loss_ref))
+    # the grid: every DM at 1e-4 over +-0.5 around the truth, in engine batches of 1024 trials
+    grid = W["true_dm"] + np.arange(-W["start_halfwidth"], W["start_halfwidth"] + 1e-12, W["grid_step"])
+    t0 = time.perf_counter()
+    gl = []
+    for s0 in range(0, grid.size, 1024):
+        dms = grid[s0:s0 + 1024].astype(np.float32)
+        ge = ag.ProfileEngine(W["bw_hz"], f0, dms, W["batch"], W["nbins"], W["period_s"])
+        prof, _, _ = ge.run(rec.data_ptr(), N, False)
+        gl.append(ag.snr_loss(torch.from_numpy(prof), noise_var).numpy())
+        del ge
+    t_grid = time.perf_counter() - t0
+    gl = np.concatenate(gl)
+    gbest = int(np.argmin(gl))
+    out = {
+        "metric": "DM refinement", "workload": W["workload"], "config": W,
+        "refined_best_dm": float(dm_ref[best]), "refined_best_loss": float(loss_ref[best]),
+        "grid_best_dm": float(grid[gbest]), "grid_best_loss": float(gl[gbest]),
+        "abs_dm_difference": abs(float(dm_ref[best]) - float(grid[gbest])),
+        "refine_seconds": t_ref, "refine_steps": steps, "refine_dm_passes": W["num_starts"] * steps * 3,
+        "grid_seconds": t_grid, "grid_trials": int(grid.size),
+        "refined_dm_spread": [float(np.min(dm_ref)), float(np.max(dm_ref))],
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -31,7 +31,8 @@ enum {
     MSOSM_ERR_HIP = 2,         /* HIP runtime failure */
     MSOSM_ERR_STATE = 3,       /* call not valid in the object's current mode */
     MSOSM_ERR_UNSUPPORTED = 4, /* valid request this build does not implement (e.g. FFT length) */
-    MSOSM_ERR_ALLOC = 5        /* host or device allocation failed */
+    MSOSM_ERR_ALLOC = 5,       /* host or device allocation failed */
+    MSOSM_ERR_IO = 6           /* file I/O failed (output pipeline) */
 };
 
 /* Input sample formats */
@@ -234,6 +235,40 @@ int msosm_multi_fold_read(msosm_multi *multi, float *profile, double *sums, unsi
 int msosm_multi_synchronize(msosm_multi *multi);
 int msosm_multi_profile(msosm_multi *multi, int enable);
 int msosm_multi_profile_read(msosm_multi *multi, int kernel, double *total_ms, long long *launches);
+
+/* ------------------------------------------------------------------------------------------------
+ * Host output pipeline: MSOSM_DataSave_Uring (data_save/msosm_data_save_uring.cpp:40-288).
+ * slot_count slots of num_files * block_bytes bytes each; push k's block of series i goes to file i at
+ * byte offset k * block_bytes. The producer acquires the next slot in turn, fills it (a device-to-host
+ * copy on a stream, or host code), and commits it; a poll thread hands filled slots to one writer
+ * thread per slot, which writes them with io_uring (raw system calls) or pwrite, resuming short
+ * writes. An I/O error does not stop the producer or the other writes (every block is attempted);
+ * msosm_saver_join reports the first one. The reference exits on the first error instead.
+ * --------------------------------------------------------------------------------------------- */
+typedef struct msosm_saver msosm_saver;
+enum {
+    MSOSM_SAVE_URING = 1,  /* io_uring writes (else pwrite, data_save/msosm_data_save_uring.cpp:224-258) */
+    MSOSM_SAVE_DIRECT = 2, /* O_DIRECT files (:63), buffered where the file system refuses it */
+    MSOSM_SAVE_PINNED = 4  /* slots in pinned host memory (hipHostMalloc, :80): needed by commit_stream */
+};
+/* config_save + initialize_uring (:40-101); paths[num_files] name the files (see saver_file_name) */
+int msosm_saver_create(const char *const *paths, int num_files, size_t block_bytes, int slot_count,
+                       unsigned long save_count, int flags, msosm_saver **saver);
+/* start_saving (:260-268): the poll thread and the writer threads */
+int msosm_saver_start(msosm_saver *saver);
+/* copy_to_slot, first half (:103-107): wait until the next slot is free; *slot_data = its buffer */
+int msosm_saver_acquire(msosm_saver *saver, void **slot_data);
+/* copy_to_slot, second half (:108-112): the acquired slot is filled once the work queued so far on
+ * stream completes (an event is recorded on it); commit_ready: it is filled already */
+int msosm_saver_commit_stream(msosm_saver *saver, void *stream);
+int msosm_saver_commit_ready(msosm_saver *saver);
+/* join_saving (:270-288): after save_count commits, wait for every write and close the files;
+ * returns the first I/O error (MSOSM_ERR_IO) if any */
+int msosm_saver_join(msosm_saver *saver);
+int msosm_saver_destroy(msosm_saver *saver);
+/* make_filename with the DM precision rule (:9-36,56-65): <dir>/<prefix>_DM<dm[i]>.dat */
+int msosm_saver_file_name(const char *dir, const char *prefix, const float *dms, int num_dms, int i, char *buf,
+                          size_t cap);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@
 //   save  <bw> <f0> <batch> <dms.txt> <in.u16> <out.u16>       check_msosm.cpp:79-80,168-174
 //   fold  <bw> <f0> <batch> <dm> <period> <nbins> <in.u16> <profile.txt>   msosm_psrdada.cpp:101-149
 //   osm   <bw> <f0> <batch> <dm> <in.u16> <out.c64>            check_osm.cpp:66-67,127-133
+//   uring <bw> <f0> <batch> <dms.txt> <in.u16> <dir> <uring|pwrite>   msosm_save_simulated.cpp:124-149
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "fold_gpu.h"
+#include "msosm_data_save_uring.h"
 #include "msosm_gpu_batch.h"
 #include "osm_gpu_batch.h"
 
@@ -50,6 +52,25 @@ int main(int argc, char **argv)
         std::ofstream of(argv[7], std::ios::binary);
         of.write(reinterpret_cast<const char *>(output.data()), output.size() * sizeof(uint16_pair));
         std::cout << "M " << msosm.M << " delaycount " << msosm.delaycount << " calls " << count << std::endl;
+    } else if (mode == "uring") {
+        std::vector<float> dms;
+        std::ifstream df(argv[5]);
+        for (float v; df >> v;) dms.push_back(v);
+        auto input = read_u16(argv[6]);
+        MSOSM_DataSave_Uring msosm(bw, dms.data(), f0, (int)dms.size());
+        msosm.use_pwrite(std::string(argv[8]) == "pwrite");
+        msosm.initialize_uint16(0, batch);
+        const size_t process_len = (size_t)batch * msosm.M;
+        const int count = (int)(input.size() / process_len);
+        msosm.config_save(argv[7], "simulated");
+        msosm.initialize_uring(4, count);
+        msosm.start_saving();
+        for (int k = 0; k < count; k++) {
+            msosm.filter_block_uint16(input.data() + k * process_len);
+            msosm.copy_to_slot();
+        }
+        msosm.join_saving();
+        std::cout << "M " << msosm.M << " calls " << count << std::endl;
     } else if (mode == "fold") {
         const float dm = std::stof(argv[5]), period = std::stof(argv[6]);
         const unsigned long nbins = std::stoul(argv[7]);

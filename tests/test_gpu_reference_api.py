@@ -83,3 +83,33 @@ def test_plain_osm(tmp_path):
     err = rel_l2(got, ref)
     print(f"OSM L=2^18 rel-L2 {err:.2e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("writer", ["uring", "pwrite"])
+def test_data_save_uring_files(tmp_path, writer):
+    # MSOSM_DataSave_Uring with the reference harness sequence (msosm_save_simulated.cpp:124-149):
+    # each DM's file holds its uint16 series, push after push, equal to the plan's own uint16 output
+    dms = [12.5, 30.0, 50.0]
+    batch, calls = 4, 5
+    design = ms.Design(BW, F0, dms)
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_U16, 1 << (n - 1).bit_length(), BW, F0, 50.0, 0.0011, seed=24)
+    raw = raw[:n]
+    (tmp_path / "dms.txt").write_text("\n".join(str(d) for d in dms))
+    raw.astype(np.uint32).tofile(tmp_path / "in.u16")
+    out = tmp_path / "out"
+    out.mkdir()
+    _run(["uring", BW, F0, batch, "dms.txt", "in.u16", str(out), writer], tmp_path)
+    plan = ms.Plan.from_design(design, batch, in_format=ms.IN_U16, out_flags=ms.OUT_U16)
+    ref = []
+    for c in range(calls):
+        plan.push(raw[c * per:(c + 1) * per])
+        ref.append(plan.output_u16())
+    plan.close()
+    ref = np.concatenate(ref, axis=1)
+    for i, dm in enumerate(dms):
+        path = ms.saver_file_name(str(out), "simulated", dms, i)
+        got = np.fromfile(path, dtype=np.uint32)
+        print(f"  {os.path.basename(path)}: {got.size} samples", flush=True)
+        assert np.array_equal(got, ref[i])
