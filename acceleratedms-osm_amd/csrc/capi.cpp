@@ -270,6 +270,7 @@ struct msosm_plan {
     long long prof_n[2] = {0, 0};
     std::vector<char> dm_set;
     int launch_mode = MSOSM_LAUNCH_PERSISTENT;
+    int sched_wg = 0, order = MSOSM_ORDER_XCD_DM_MAJOR; // tile schedule (0 workgroups = G)
     // DM-derivative pass: dH/dDM per DM and the per-push primal series it is paired with
     float2 *Hdot = nullptr, *tan_y = nullptr;
     std::vector<char> tan_set;
@@ -541,6 +542,7 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     ia.tw_lo = fa.tw_lo;
     ia.tw_hi = fa.tw_hi;
     ia.tw_s = p->tw_s;
+    ia.order = p->order;
     if (p->fold) {
         msosm_fold *f = p->fold;
         a.has_bins = true;
@@ -666,7 +668,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
                 HIPCHK(launch_inv(p->log2L, one, std::min(p->B, p->G), p->stream));
             }
         } else {
-            HIPCHK(launch_inv(p->log2L, x, p->G, p->stream));
+            HIPCHK(launch_inv(p->log2L, x, p->sched_wg ? p->sched_wg : p->G, p->stream));
         }
         return MSOSM_OK;
     };
@@ -813,6 +815,79 @@ extern "C" int msosm_plan_profile_read(msosm_plan *p, int kernel, double *total_
     if (collect_profile(p) != MSOSM_OK) return MSOSM_ERR_HIP;
     if (total_ms) *total_ms = p->prof_ms[kernel];
     if (launches) *launches = p->prof_n[kernel];
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_set_schedule(msosm_plan *p, int workgroups, int order)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_set_schedule: null plan");
+    if (workgroups < 0 || workgroups > p->G)
+        return set_error(MSOSM_ERR_ARG, "plan_set_schedule: workgroups must be in [0, " + std::to_string(p->G) + "]");
+    if (order < MSOSM_ORDER_XCD_DM_MAJOR || order > MSOSM_ORDER_XCD_BLOCK_MAJOR)
+        return set_error(MSOSM_ERR_ARG, "plan_set_schedule: unknown order");
+    p->sched_wg = workgroups;
+    p->order = order;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_get_schedule(msosm_plan *p, int *workgroups, int *order)
+{
+    if (!p) return set_error(MSOSM_ERR_ARG, "plan_get_schedule: null plan");
+    if (workgroups) *workgroups = p->sched_wg ? p->sched_wg : p->G;
+    if (order) *order = p->order;
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_autotune(msosm_plan *p, const void *input, int pushes, int *best_wg, int *best_order,
+                                   double *best_ms)
+{
+    if (!p || !input) return set_error(MSOSM_ERR_ARG, "plan_autotune: null argument");
+    if (pushes < 2) return set_error(MSOSM_ERR_ARG, "plan_autotune: need at least 2 pushes of input");
+    if (p->launch_mode != MSOSM_LAUNCH_PERSISTENT)
+        return set_error(MSOSM_ERR_STATE, "plan_autotune: the schedule applies to the persistent launch only");
+    HIPCHK(hipSetDevice(p->device));
+    const bool was_profiling = p->profile;
+    if (collect_profile(p) != MSOSM_OK) return MSOSM_ERR_HIP;
+    const size_t step = (size_t)p->B * p->M * p->in_bytes;
+    const int half = std::max(8, (p->G / 2 / 8) * 8);
+    const int wgs[2] = {p->G, half};
+    double best = 1e300;
+    int bw = p->G, bo = MSOSM_ORDER_XCD_DM_MAJOR;
+    for (int wi = 0; wi < (half < p->G ? 2 : 1); ++wi)
+        for (int o = MSOSM_ORDER_XCD_DM_MAJOR; o <= MSOSM_ORDER_XCD_BLOCK_MAJOR; ++o) {
+            p->sched_wg = wgs[wi];
+            p->order = o;
+            int rc = msosm_plan_reset(p);
+            if (rc == MSOSM_OK && p->fold) rc = msosm_fold_reset(p->fold);
+            if (rc == MSOSM_OK) rc = push_impl(p, input, MSOSM_INPUT_DEVICE, nullptr);
+            if (rc == MSOSM_OK) rc = collect_profile(p);
+            if (rc != MSOSM_OK) return rc;
+            p->profile = true;
+            p->prof_ms[1] = 0;
+            p->prof_n[1] = 0;
+            for (int k = 1; k < pushes && rc == MSOSM_OK; ++k)
+                rc = push_impl(p, static_cast<const char *>(input) + k * step, MSOSM_INPUT_DEVICE, nullptr);
+            if (rc == MSOSM_OK) rc = collect_profile(p);
+            p->profile = false;
+            if (rc != MSOSM_OK) return rc;
+            const double ms = p->prof_ms[1] / (double)std::max(1LL, p->prof_n[1]);
+            if (ms < best) {
+                best = ms;
+                bw = wgs[wi];
+                bo = o;
+            }
+        }
+    p->sched_wg = bw == p->G ? 0 : bw;
+    p->order = bo;
+    int rc = msosm_plan_reset(p);
+    if (rc == MSOSM_OK && p->fold) rc = msosm_fold_reset(p->fold);
+    if (rc != MSOSM_OK) return rc;
+    p->profile = was_profiling;
+    p->prof_ms[0] = p->prof_ms[1] = 0;
+    p->prof_n[0] = p->prof_n[1] = 0;
+    if (best_wg) *best_wg = bw;
+    if (best_order) *best_order = bo;
+    if (best_ms) *best_ms = best;
     return MSOSM_OK;
 }
 

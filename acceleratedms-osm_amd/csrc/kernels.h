@@ -40,6 +40,7 @@ struct InvArgs {
     int nbins;
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
+    int order;                // unit order (MSOSM_ORDER_*): which units run together
 };
 
 // One persistent launch over several FFT lengths ("length classes"): class c owns the units

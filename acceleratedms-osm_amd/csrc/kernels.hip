@@ -54,6 +54,8 @@ template <int N1, int N2, bool RING32, int VAR>
 __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
 {
     using S = Shape<N1, N2>;
+    // tile prefetch where its 32 registers fit without spills (plain epilogue, L <= 2^16)
+    constexpr bool PREFETCH = VAR == VAR_PLAIN && S::L <= (1 << 16);
     extern __shared__ float2 lds_all[];
     Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
     load_twiddles<N1, N2>(lds_all, tw);
@@ -62,14 +64,21 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
-    // XCD-aware virtual id: workgroups w, w+8, ... share an XCD; give them consecutive virtual ids
+    // XCD-aware virtual id: workgroups w, w+8, ... share an XCD; give them consecutive virtual ids, so
+    // that the workgroups of one XCD (and its L2) run consecutive units together
     const int G = gridDim.x;
     const int w = blockIdx.x;
-    const int v = (G % 8 == 0) ? ((w % 8) * (G / 8) + w / 8) : w;
+    const bool xcd = a.order != MSOSM_ORDER_LINEAR && G % 8 == 0;
+    const int v = xcd ? ((w % 8) * (G / 8) + w / 8) : w;
     const long long U = (long long)a.D * a.B;
+    const bool block_major = a.order == MSOSM_ORDER_XCD_BLOCK_MAJOR;
 #pragma unroll 1
-    for (long long u = v; u < U; u += G)
-        inv_unit<N1, N2, RING32, VAR>(a, (int)(u / a.B), (int)(u % a.B), lds_all, tw, scr, hsum, hcnt, pw_lds);
+    for (long long u = v; u < U; u += G) {
+        // DM-major: consecutive units share a DM's chirp and delays; block-major: they share ring slots
+        const int d = block_major ? (int)(u % a.D) : (int)(u / a.B);
+        const int b = block_major ? (int)(u / a.D) : (int)(u % a.B);
+        inv_unit<N1, N2, RING32, VAR, PREFETCH>(a, d, b, lds_all, tw, scr, hsum, hcnt, pw_lds);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

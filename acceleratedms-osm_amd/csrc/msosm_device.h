@@ -99,13 +99,41 @@ __device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
 // Pass A: column sub-FFTs. load(k) returns input element k (k = k1 + N1*k2).
 // Output Z[k1][n2] * W^{DIR n2 k1} goes to scr[(tile * N2 + n2) * NCA + col].
 // ---------------------------------------------------------------------------------------------
-template <int N1, int N2, int DIR, class Load>
+// With PF, the next tile's inputs are loaded while the current tile is transformed (32 more VGPRs):
+// the loads' latency then overlaps the sub-FFT instead of stalling every wave at the tile start.
+template <int N1, int N2, int DIR, bool PF = false, class Load>
 __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ scr, float2 *lds_all, const Tw &tw)
 {
     using S = Shape<N1, N2>;
     constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
+    auto fetch = [&](int t0, float2 *dst) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int col = lt % NC, jt = lt / NC;
+        const int tile = t0 + sub;
+        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA; // all sub-tiles busy when divisible
+        const int k1 = tile * NC + col;
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) dst[t] = load(k1 + N1 * (jt + t * TPC));
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) dst[t] = make_float2(0.f, 0.f);
+        }
+    };
+    float2 nx[16];
+    if constexpr (PF) fetch(0, nx);
 #pragma unroll 1
     for (int t0 = 0; t0 < S::NTA; t0 += SUBT) {
+        float2 v[16];
+        if constexpr (PF) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = nx[t];
+            if (t0 + SUBT < S::NTA) fetch(t0 + SUBT, nx);
+        } else {
+            fetch(t0, v);
+        }
         // opaque thread index: keeps per-thread addresses from being hoisted (and spilled) across tiles
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
@@ -113,16 +141,8 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
         const int col = lt % NC, jt = lt / NC;
         float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
-        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA; // all sub-tiles busy when divisible
+        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA;
         const int k1 = tile * NC + col;
-        float2 v[16];
-        if (active) {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) v[t] = load(k1 + N1 * (jt + t * TPC));
-        } else {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) v[t] = make_float2(0.f, 0.f);
-        }
         subfft<NS, NC, true, DIR>(v, lds, col, jt, tw.n2);
         if (active) {
             // element (q, t) is output n2 = jt + q*TPC + t*NsL; its twiddle W^{n2 k1} is
@@ -150,13 +170,42 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
 // f = n1 * NCB + row, n2 = tile * NCB + row. Epilogues read f = lane + 256 * i, so consecutive
 // lanes hold consecutive samples (runs of NCB) and neither the LDS reads nor the stores conflict.
 // ---------------------------------------------------------------------------------------------
-template <int N1, int N2, int DIR, class Emit>
+template <int N1, int N2, int DIR, bool PF = false, class Emit>
 __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emit &emit, float2 *lds_all, const Tw &tw)
 {
     using S = Shape<N1, N2>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
+    auto fetch = [&](int t0, float2 *dst) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int jt = lt % TPC, row = lt / TPC;
+        const int tile = t0 + sub;
+        const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
+        const int n2 = tile * NC + row;
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int k1 = jt + t * TPC;
+                dst[t] = ld32(scr, (uint32_t)(((k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) dst[t] = make_float2(0.f, 0.f);
+        }
+    };
+    float2 nx[16];
+    if constexpr (PF) fetch(0, nx);
 #pragma unroll 1
     for (int t0 = 0; t0 < S::NTB; t0 += SUBT) {
+        float2 v[16];
+        if constexpr (PF) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = nx[t];
+            if (t0 + SUBT < S::NTB) fetch(t0 + SUBT, nx);
+        } else {
+            fetch(t0, v);
+        }
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int sub = tid >> 8, lt = tid & 255;
@@ -164,18 +213,6 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
         float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
         const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
-        const int n2 = tile * NC + row;
-        float2 v[16];
-        if (active) {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const int k1 = jt + t * TPC;
-                v[t] = ld32(scr, (uint32_t)(((k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)));
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) v[t] = make_float2(0.f, 0.f);
-        }
         subfft<NS, NC, false, DIR>(v, lds, row, jt, tw.n1);
         __syncthreads();
         {
@@ -246,7 +283,7 @@ constexpr int VAR_PLAIN = 0, VAR_F16 = 1, VAR_TANGENT = 2;
 // One dedispersion unit (DM d, block b): gather * H -> IDFT_L -> keep [M, L) * 1/L -> outputs and
 // fold. hsum/hcnt are the workgroup's LDS fold histogram (zero on entry and on exit), pw_lds the
 // FP16-scale reduction word (zero on entry and on exit).
-template <int N1, int N2, bool RING32, int VAR>
+template <int N1, int N2, bool RING32, int VAR, bool PF = false>
 __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
                                          float2 *__restrict__ scr, double *hsum, unsigned int *hcnt, float *pw_lds)
 {
@@ -274,7 +311,7 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
         if constexpr (f16) power += y.x * y.x + y.y * y.y;
         return y;
     };
-    pass_a<N1, N2, +1>(load, scr, lds_all, tw);
+    pass_a<N1, N2, +1, PF>(load, scr, lds_all, tw);
     float inv_scale = 1.f, out_scale = 1.f;
     if constexpr (f16) {
         // block rms of the 1/L-scaled series: sqrt(sum |Y|^2) / L; scale = 2^floor(log2(rms))
@@ -360,7 +397,7 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
             atomicAdd(&hcnt[cur], ccnt);
         }
     };
-    pass_b<N1, N2, +1>(scr, emit, lds_all, tw);
+    pass_b<N1, N2, +1, PF>(scr, emit, lds_all, tw);
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f; // next unit's reduction (ordered by the barriers)
     if (fold) {

@@ -42,8 +42,10 @@ EXPORTS = [
     "msosm_multi_set_graph",
     "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
+    "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
+ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR = 0, 1, 2
 
 
 class MsosmError(RuntimeError):
@@ -147,6 +149,9 @@ def lib():
         "msosm_saver_join": ([vp], C.c_int),
         "msosm_saver_destroy": ([vp], C.c_int),
         "msosm_saver_file_name": ([C.c_char_p, C.c_char_p, fp, C.c_int, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
+        "msosm_plan_set_schedule": ([vp, C.c_int, C.c_int], C.c_int),
+        "msosm_plan_get_schedule": ([vp, ip, ip], C.c_int),
+        "msosm_plan_autotune": ([vp, vp, C.c_int, ip, ip, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -396,6 +401,20 @@ class Plan:
     def attach_fold(self, fold: Fold | None):
         _check(lib().msosm_plan_attach_fold(self.h, fold.h if fold else None))
         self.fold = fold
+
+    def set_schedule(self, workgroups: int = 0, order: int = 0) -> None:
+        _check(lib().msosm_plan_set_schedule(self.h, workgroups, order))
+
+    def schedule(self) -> tuple[int, int]:
+        w, o = C.c_int(), C.c_int()
+        _check(lib().msosm_plan_get_schedule(self.h, C.byref(w), C.byref(o)))
+        return w.value, o.value
+
+    def autotune(self, input_dev: int, pushes: int = 3) -> tuple[int, int, float]:
+        """Time every schedule on `pushes` pushes of device input; apply the fastest (resets the plan)."""
+        w, o, t = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().msosm_plan_autotune(self.h, C.c_void_p(input_dev), pushes, C.byref(w), C.byref(o), C.byref(t)))
+        return w.value, o.value, t.value
 
     def set_dm_tangent(self, i: int, dH: np.ndarray) -> None:
         """Upload dH/dDM of DM i for tangent pushes."""

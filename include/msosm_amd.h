@@ -152,6 +152,22 @@ enum {
     MSOSM_LAUNCH_PER_CLASS = 2   /* uneven grids only: one persistent launch per FFT length */
 };
 int msosm_plan_set_launch_mode(msosm_plan *plan, int mode);
+
+/* Tile schedule of the persistent dedispersion launch (no reference counterpart): the number of
+ * workgroups (0 = one per CU, rounded down to a multiple of 8; at most that) and the unit order. */
+enum {
+    MSOSM_ORDER_XCD_DM_MAJOR = 0,   /* the workgroups of one XCD run consecutive blocks of one DM (default) */
+    MSOSM_ORDER_LINEAR = 1,         /* workgroup w runs units w, w+G, ... in DM-major order */
+    MSOSM_ORDER_XCD_BLOCK_MAJOR = 2 /* as the default, but consecutive units are one block of consecutive DMs */
+};
+int msosm_plan_set_schedule(msosm_plan *plan, int workgroups, int order);
+int msosm_plan_get_schedule(msosm_plan *plan, int *workgroups, int *order);
+/* Autotuner: times the dedispersion launch of every schedule candidate (unit orders x workgroup
+ * counts {max, max/2}) over `pushes` pushes of device input (batch*M samples each, in the plan's
+ * input format; the first push of each candidate is not timed), applies the fastest and returns it.
+ * The tuning pushes consume the plan's state: the plan and its attached fold are reset afterwards. */
+int msosm_plan_autotune(msosm_plan *plan, const void *input_dev, int pushes, int *best_workgroups,
+                        int *best_order, double *best_ms);
 /* HIP-event timing of the plan's kernels. kernel: 0 = forward FFT, 1 = dedispersion */
 int msosm_plan_profile(msosm_plan *plan, int enable);
 int msosm_plan_profile_read(msosm_plan *plan, int kernel, double *total_ms, long long *launches);

@@ -8,7 +8,7 @@ import pytest
 
 import msosm_amd as ms
 from oracle import oracle as orc
-from tests.util import make_input, oracle_design, rel_l2
+from tests.util import check_u16_codes, make_input, oracle_design, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -42,11 +42,8 @@ def test_save_mode_uint16(tmp_path):
     got = got.reshape(calls, len(dms), per).transpose(1, 0, 2).reshape(len(dms), n)
     delays, H = oracle_design(BW, F0, design.dms, design.M)
     ref = orc.msosm(x, design.M, delays, H)
-    enc = np.vectorize(orc.encode_u16_pair, otypes=[np.uint32])
-    ref_codes = enc(ref.real.astype(np.float32), ref.imag.astype(np.float32))
-    diff = np.maximum(np.abs((got & 0xFFFF).astype(np.int64) - (ref_codes & 0xFFFF)),
-                      np.abs((got >> 16).astype(np.int64) - (ref_codes >> 16)))
-    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    frac, worst = check_u16_codes(got, ref)
+    print(f"  codes differing (all at rounding ties) {frac:.2e}, max diff {worst}", flush=True)
 
 
 def test_fold_mode_profile(tmp_path):

@@ -49,3 +49,25 @@ def gpu_series(design: ms.Design, raw: np.ndarray, fmt: int, batch: int, out=ms.
         outs.append(plan.output_c64() if out & ms.OUT_C64 else plan.output_u16())
     plan.close()
     return np.concatenate(outs, axis=1)
+
+
+def check_u16_codes(got: np.ndarray, ref: np.ndarray, tie_window: float = 0.02):
+    """uint16 codes from the GPU against the FP64 series: every code equals the oracle's encoding of the
+    FP64 value, except where that value lies within tie_window of a rounding boundary (x + 32768 at
+    k + 1/2), where FP32 rounding may legitimately pick the neighbour; those differ by one code.
+    Returns (fraction of components that differ, max code difference)."""
+    enc = np.vectorize(orc.encode_u16_pair, otypes=[np.uint32])
+    ref_codes = enc(ref.real.astype(np.float32), ref.imag.astype(np.float32))
+    worst, ndiff, ntot = 0, 0, 0
+    for shift, comp in ((0, ref.real), (16, ref.imag)):
+        g = ((got >> shift) & 0xFFFF).astype(np.int64)
+        r = ((ref_codes >> shift) & 0xFFFF).astype(np.int64)
+        d = np.abs(g - r)
+        worst = max(worst, int(d.max()))
+        v = comp + np.where(comp != 0, 32768.0, 0.0)
+        near_tie = np.abs(v - np.floor(v) - 0.5) < tie_window
+        assert np.all(near_tie[d > 0]), "a code differs away from a rounding boundary"
+        ndiff += int((d > 0).sum())
+        ntot += d.size
+    assert worst <= 1
+    return ndiff / ntot, worst
