@@ -166,6 +166,7 @@ struct msosm_saver {
     std::string err;
     int err_code = MSOSM_OK;
     std::atomic<bool> failed{false};
+    bool direct_fallback = false; // O_DIRECT requested but refused for the slot memory
 };
 
 static void record_failure(msosm_saver *s, int code, const std::string &msg)
@@ -405,6 +406,25 @@ extern "C" int msosm_saver_create(const char *const *paths, int num_files, size_
     }
     s->has_event.assign(slot_count, 0);
     s->bad.assign(slot_count, 0);
+    if (flags & MSOSM_SAVE_DIRECT) {
+        // Direct I/O pins the source pages in the kernel; pinned HIP host memory is a device mapping
+        // that it cannot pin here (the write fails with EFAULT). Probe one block on the first file
+        // and fall back to buffered files when direct writes from the slots are refused.
+        const size_t probe = std::min<size_t>(block_bytes, 4096);
+        memset(s->slots, 0, probe);
+        const ssize_t w = pwrite(s->fds[0], s->slots, probe, 0);
+        if (w < 0 && (errno == EFAULT || errno == EINVAL)) {
+            s->direct_fallback = true;
+            for (int i = 0; i < num_files; ++i) {
+                ::close(s->fds[i]);
+                s->fds[i] = open(paths[i], O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+                if (s->fds[i] < 0)
+                    return fail(MSOSM_ERR_IO, std::string("saver_create: open ") + s->paths[i] + ": " + strerror(errno));
+            }
+        } else if (w < 0 || ftruncate(s->fds[0], 0) != 0) {
+            return fail(MSOSM_ERR_IO, std::string("saver_create: probe write ") + s->paths[0] + ": " + strerror(errno));
+        }
+    }
     if (flags & MSOSM_SAVE_URING) {
         s->rings.resize(slot_count);
         const unsigned entries = (unsigned)std::min(4096, std::max(8, num_files));

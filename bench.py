@@ -45,7 +45,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(design, n_dm=24, n_samples=1 << 22):
+def cpu_baseline(design, n_dm=64, n_samples=1 << 22):
     """FP64 oracle (single thread) on a bounded sample: the first n_dm trials of the same grid over
     n_samples samples, dedispersed and folded exactly as the GPU path."""
     import numpy as np

@@ -264,7 +264,8 @@ int msosm_multi_profile_read(msosm_multi *multi, int kernel, double *total_ms, l
 typedef struct msosm_saver msosm_saver;
 enum {
     MSOSM_SAVE_URING = 1,  /* io_uring writes (else pwrite, data_save/msosm_data_save_uring.cpp:224-258) */
-    MSOSM_SAVE_DIRECT = 2, /* O_DIRECT files (:63), buffered where the file system refuses it */
+    MSOSM_SAVE_DIRECT = 2, /* O_DIRECT files (:63); buffered where the file system, or the kernel for the
+                              slot memory (pinned HIP host memory cannot be pinned for direct I/O), refuses it */
     MSOSM_SAVE_PINNED = 4  /* slots in pinned host memory (hipHostMalloc, :80): needed by commit_stream */
 };
 /* config_save + initialize_uring (:40-101); paths[num_files] name the files (see saver_file_name) */
