@@ -429,6 +429,15 @@ extern "C" int msosm_plan_destroy(msosm_plan *p)
     return MSOSM_OK;
 }
 
+// copy L complex values to the device, multiplied by 1/L (exact for normal values: a power of two)
+static hipError_t upload_scaled(float2 *dst, const float *src, int L)
+{
+    std::vector<float> tmp(2 * (size_t)L);
+    const float s = std::ldexp(1.0f, -ilog2(L));
+    for (size_t j = 0; j < tmp.size(); ++j) tmp[j] = src[j] * s;
+    return hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice);
+}
+
 extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, const float *H)
 {
     if (!p || !delay_points || !H) return set_error(MSOSM_ERR_ARG, "plan_set_dm: null argument");
@@ -444,7 +453,8 @@ extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, 
     HIPCHK(hipSetDevice(p->device));
     HIPCHK(hipStreamSynchronize(p->stream));
     HIPCHK(hipMemcpy(p->delay + (size_t)i * p->L, dl.data(), p->L * sizeof(uint16_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(p->H + (size_t)i * p->L, H, p->L * sizeof(float2), hipMemcpyHostToDevice));
+    // the device copy carries the inverse transform's 1/L (exact: a power of two)
+    HIPCHK(upload_scaled(p->H + (size_t)i * p->L, H, p->L));
     p->dm_set[i] = 1;
     return MSOSM_OK;
 }
@@ -464,7 +474,7 @@ extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
             return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
         p->tan_set.assign(p->D, 0);
     }
-    HIPCHK(hipMemcpy(p->Hdot + (size_t)i * p->L, dH, p->L * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(upload_scaled(p->Hdot + (size_t)i * p->L, dH, p->L));
     p->tan_set[i] = 1;
     return MSOSM_OK;
 }
@@ -823,7 +833,7 @@ extern "C" int msosm_plan_set_schedule(msosm_plan *p, int workgroups, int order)
     if (!p) return set_error(MSOSM_ERR_ARG, "plan_set_schedule: null plan");
     if (workgroups < 0 || workgroups > p->G)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: workgroups must be in [0, " + std::to_string(p->G) + "]");
-    if (order < MSOSM_ORDER_XCD_DM_MAJOR || order > MSOSM_ORDER_XCD_BLOCK_MAJOR)
+    if (order < MSOSM_ORDER_XCD_DM_MAJOR || order > MSOSM_ORDER_XCD_TILE8)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: unknown order");
     p->sched_wg = workgroups;
     p->order = order;
@@ -854,7 +864,7 @@ extern "C" int msosm_plan_autotune(msosm_plan *p, const void *input, int pushes,
     double best = 1e300;
     int bw = p->G, bo = MSOSM_ORDER_XCD_DM_MAJOR;
     for (int wi = 0; wi < (half < p->G ? 2 : 1); ++wi)
-        for (int o = MSOSM_ORDER_XCD_DM_MAJOR; o <= MSOSM_ORDER_XCD_BLOCK_MAJOR; ++o) {
+        for (int o = MSOSM_ORDER_XCD_DM_MAJOR; o <= MSOSM_ORDER_XCD_TILE8; ++o) {
             p->sched_wg = wgs[wi];
             p->order = o;
             int rc = msosm_plan_reset(p);

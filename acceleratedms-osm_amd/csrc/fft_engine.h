@@ -75,7 +75,7 @@ __device__ __forceinline__ void dft(float2 *v)
         dft4<DIR>(v[0], v[2], v[4], v[6]);
         dft4<DIR>(v[1], v[3], v[5], v[7]);
         v[3] = cmul(v[3], w16<DIR, 2>());
-        v[5] = cmul(v[5], w16<DIR, 4>());
+        v[5] = mul_di<DIR>(v[5]); // w16^4 = DIR * i: a swap and a sign, no multiplies
         v[7] = cmul(v[7], w16<DIR, 6>());
         dft2<DIR>(v[0], v[1]);
         dft2<DIR>(v[2], v[3]);
@@ -97,7 +97,7 @@ __device__ __forceinline__ void dft(float2 *v)
         v[9] = cmul(v[9], w16<DIR, 2>());
         v[13] = cmul(v[13], w16<DIR, 3>());
         v[6] = cmul(v[6], w16<DIR, 2>());
-        v[10] = cmul(v[10], w16<DIR, 4>());
+        v[10] = mul_di<DIR>(v[10]);
         v[14] = cmul(v[14], w16<DIR, 6>());
         v[7] = cmul(v[7], w16<DIR, 3>());
         v[11] = cmul(v[11], w16<DIR, 6>());

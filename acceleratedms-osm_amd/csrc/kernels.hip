@@ -68,16 +68,35 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     // that the workgroups of one XCD (and its L2) run consecutive units together
     const int G = gridDim.x;
     const int w = blockIdx.x;
+    // unit orders (one loop, so that the unit body is inlined once):
+    //   tiles: the G/8 workgroups of an XCD run dg DMs x (G/8)/dg consecutive blocks at a time, so each
+    //          DM's chirp and delays and each block's ring slots are shared in that XCD's L2;
+    //   XCD DM-/block-major: workgroups w, w+8, ... share an XCD and get consecutive virtual ids;
+    //   linear: workgroup w runs units w, w+G, ...
+    const int dg = a.order == MSOSM_ORDER_XCD_TILE4 ? 4 : a.order == MSOSM_ORDER_XCD_TILE8 ? 8 : 0;
+    const bool tiles = dg && G % 8 == 0 && (G / 8) % dg == 0;
     const bool xcd = a.order != MSOSM_ORDER_LINEAR && G % 8 == 0;
-    const int v = xcd ? ((w % 8) * (G / 8) + w / 8) : w;
-    const long long U = (long long)a.D * a.B;
     const bool block_major = a.order == MSOSM_ORDER_XCD_BLOCK_MAJOR;
+    const int bg = tiles ? (G / 8) / dg : 1, j = w / 8;
+    const int tb = (a.B + bg - 1) / bg, td = tiles ? (a.D + dg - 1) / dg : 0;
+    const long long first = tiles ? w % 8 : xcd ? (w % 8) * (G / 8) + w / 8 : w;
+    const long long end = tiles ? (long long)td * tb : (long long)a.D * a.B;
+    const long long step = tiles ? 8 : G;
 #pragma unroll 1
-    for (long long u = v; u < U; u += G) {
-        // DM-major: consecutive units share a DM's chirp and delays; block-major: they share ring slots
-        const int d = block_major ? (int)(u % a.D) : (int)(u / a.B);
-        const int b = block_major ? (int)(u / a.D) : (int)(u % a.B);
-        inv_unit<N1, N2, RING32, VAR, PREFETCH>(a, d, b, lds_all, tw, scr, hsum, hcnt, pw_lds);
+    for (long long s = first; s < end; s += step) {
+        int d, b;
+        if (tiles) {
+            d = (int)(s / tb) * dg + j / bg;
+            b = (int)(s % tb) * bg + j % bg;
+        } else if (block_major) {
+            d = (int)(s % a.D);
+            b = (int)(s / a.D);
+        } else {
+            d = (int)(s / a.B);
+            b = (int)(s % a.B);
+        }
+        if (d < a.D && b < a.B) // uniform over the workgroup
+            inv_unit<N1, N2, RING32, VAR, PREFETCH>(a, d, b, lds_all, tw, scr, hsum, hcnt, pw_lds);
     }
 }
 

@@ -6,7 +6,7 @@
 //           unpack, or the per-bin ring gather times the chirp H), then the W_L^{n2 k1} twiddle,
 //           into the workgroup's private scratch slot;
 //   pass B: N2 row sub-FFTs of length N1 from the scratch slot, fused with the consumer (ring
-//           store, or discard + 1/L scale + fold / complex64 / uint16 / FP16 output).
+//           store, or discard + fold / complex64 / uint16 / FP16 output; 1/L rides in H).
 // A workgroup keeps four 4096-point tiles in flight (256 threads each).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -292,7 +292,8 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
     const bool fold = a.fold_sum != nullptr;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
-    const float scale = 1.0f / (float)L;
+    // a.H (and dH/dDM) carry the 1/L of the inverse transform: a power of two, so the series is the
+    // same bits as scaling after the transform, one multiply per sample cheaper
     // opaque thread index: keeps thread-dependent values from being hoisted out of the caller's
     // unit loop (where they would stay live, and spill, across units)
     int tid = threadIdx.x;
@@ -302,7 +303,7 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
     const float2 *__restrict__ Hd = a.H + (size_t)d * L;
     const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring;
-    float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval)
+    float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval; Y includes 1/L)
     auto load = [&](int k) -> float2 {
         int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
         if (src < 0) src += FBS;
@@ -314,13 +315,14 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
     pass_a<N1, N2, +1, PF>(load, scr, lds_all, tw);
     float inv_scale = 1.f, out_scale = 1.f;
     if constexpr (f16) {
-        // block rms of the 1/L-scaled series: sqrt(sum |Y|^2) / L; scale = 2^floor(log2(rms))
+        // block rms of the series over the L outputs: sqrt(sum |Y|^2) with Y = X H / L (Parseval);
+        // scale = 2^floor(log2(rms))
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
         if ((tid & 63) == 0) atomicAdd(pw_lds, power);
         __syncthreads();
         const float tot = *pw_lds;
-        const float rms = sqrtf(tot) * scale;
+        const float rms = sqrtf(tot);
         int e2 = 0;
         (void)frexpf(rms, &e2); // rms = m * 2^e2, m in [0.5, 1)
         out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
@@ -361,8 +363,6 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
             const int n1 = f / NC, row = f % NC;
             const int m = tile * NC + row + N2 * n1 - M;
             float2 y = lds[lds_pad(f)];
-            y.x *= scale;
-            y.y *= scale;
             if (oc) st32(oc, (uint32_t)m, y);
             if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
             if constexpr (f16)

@@ -45,7 +45,7 @@ EXPORTS = [
     "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
-ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR = 0, 1, 2
+ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
 
 
 class MsosmError(RuntimeError):

@@ -158,7 +158,9 @@ int msosm_plan_set_launch_mode(msosm_plan *plan, int mode);
 enum {
     MSOSM_ORDER_XCD_DM_MAJOR = 0,   /* the workgroups of one XCD run consecutive blocks of one DM (default) */
     MSOSM_ORDER_LINEAR = 1,         /* workgroup w runs units w, w+G, ... in DM-major order */
-    MSOSM_ORDER_XCD_BLOCK_MAJOR = 2 /* as the default, but consecutive units are one block of consecutive DMs */
+    MSOSM_ORDER_XCD_BLOCK_MAJOR = 2, /* as the default, but consecutive units are one block of consecutive DMs */
+    MSOSM_ORDER_XCD_TILE4 = 3,       /* an XCD's workgroups run 4 DMs x (workgroups/32) blocks at a time */
+    MSOSM_ORDER_XCD_TILE8 = 4        /* the same with 8 DMs */
 };
 int msosm_plan_set_schedule(msosm_plan *plan, int workgroups, int order);
 int msosm_plan_get_schedule(msosm_plan *plan, int *workgroups, int *order);

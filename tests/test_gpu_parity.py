@@ -203,7 +203,8 @@ def test_schedules_are_bitwise_identical_and_autotune_resets():
     raw = np.ascontiguousarray(raw[:n])
     dev = torch.from_numpy(raw.view(np.float32)).cuda()
     outs = []
-    for wg, order in [(0, ms.ORDER_XCD_DM_MAJOR), (0, ms.ORDER_LINEAR), (0, ms.ORDER_XCD_BLOCK_MAJOR), (16, 0)]:
+    for wg, order in [(0, ms.ORDER_XCD_DM_MAJOR), (0, ms.ORDER_LINEAR), (0, ms.ORDER_XCD_BLOCK_MAJOR),
+                      (0, ms.ORDER_XCD_TILE4), (0, ms.ORDER_XCD_TILE8), (16, 0), (64, ms.ORDER_XCD_TILE8)]:
         plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
         plan.set_schedule(wg, order)
         o = []
