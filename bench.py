@@ -27,7 +27,7 @@ HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 # 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
 # that DM, 2^14..2^18 here), complex64 series out; one launch over all lengths vs one per DM.
 UNEVEN = dict(workload="msosm_uneven_c64", samples=1 << 26, fs_hz=32e6, fc_hz=800e6, bw_hz=32e6, num_dms=1024,
-              dm_lo=0.0, dm_hi=1000.0, push_blocks=1, output="complex64 series", input="complex64, normal")
+              dm_lo=0.0, dm_hi=1000.0, push_blocks=4, output="complex64 series", input="complex64, normal")
 
 # DM refinement (--workload refine): complex64 baseband, 2^24 samples at 16 MS/s, band 1392-1408 MHz,
 # a dispersed pulsar (DM 123.456, P = 33 ms, Gaussian pulse of 3 % FWHM, amplitude 0.5, AWGN sigma 1),
@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
     ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
     ap.add_argument("--refine-steps", type=int, default=40)
+    ap.add_argument("--push-blocks", type=int, default=0,
+                    help="uneven workload: blocks of the longest length per push (default 4)")
     ap.add_argument("--plain-pushes", type=int, default=8,
                     help="uneven workload: pushes timed in the one-launch-per-DM form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -202,7 +204,9 @@ def bench_uneven(args):
 
     import msosm_amd as ms
 
-    W = UNEVEN
+    W = dict(UNEVEN)
+    if args.push_blocks:
+        W["push_blocks"] = args.push_blocks
     f0 = W["fc_hz"] - W["bw_hz"] / 2
     dms = np.linspace(W["dm_lo"], W["dm_hi"], W["num_dms"]).astype(np.float32)
     t = time.perf_counter()
