@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--refine-steps", type=int, default=40)
     ap.add_argument("--push-blocks", type=int, default=0,
                     help="uneven workload: blocks of the longest length per push (default 4)")
+    ap.add_argument("--graph", action="store_true", help="uneven workload: run each one-launch push as a hipGraph")
     ap.add_argument("--plain-pushes", type=int, default=8,
                     help="uneven workload: pushes timed in the one-launch-per-DM form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -233,6 +234,7 @@ def bench_uneven(args):
     for name, mode, n in (("one_launch", ms.LAUNCH_PERSISTENT, pushes),
                           ("per_dm", ms.LAUNCH_PER_DM, min(args.plain_pushes, pushes))):
         multi.set_launch_mode(mode)
+        multi.set_graph(args.graph and mode == ms.LAUNCH_PERSISTENT)
         run(2)
         multi.profile(True)
         t0 = time.perf_counter()
@@ -248,6 +250,7 @@ def bench_uneven(args):
             f"{kn} dedispersion launches, {kms:.1f} ms in them")
     one = res["one_launch"]
     alg = D * per * 24.0  # 16 B gathered spectrum + 8 B complex64 output per DM-sample
+    # (with --graph the HIP events bracket the whole push graph, forward transforms included)
     avg_s = one["kernel_ms"] / one["launches"] / 1e3
     roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": alg / avg_s / HBM_PEAK, "traffic": None, "kernel": "inv_multi_kernel",
@@ -256,7 +259,7 @@ def bench_uneven(args):
         "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 2, "ms_per_step": one["ms_per_push"] * pushes, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": dict(W, length_classes=lengths), "roofline": roof, "cpu_baseline": None,
+        "config": dict(W, length_classes=lengths, graph=bool(args.graph)), "roofline": roof, "cpu_baseline": None,
         "per_dm_launch_form": {"value": res["per_dm"]["rate"], "pushes": res["per_dm"]["pushes"],
                                "speedup_of_one_launch": one["rate"] / res["per_dm"]["rate"]},
     }

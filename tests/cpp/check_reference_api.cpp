@@ -5,6 +5,7 @@
 //   fold  <bw> <f0> <batch> <dm> <period> <nbins> <in.u16> <profile.txt>   msosm_psrdada.cpp:101-149
 //   osm   <bw> <f0> <batch> <dm> <in.u16> <out.c64>            check_osm.cpp:66-67,127-133
 //   uring <bw> <f0> <batch> <dms.txt> <in.u16> <dir> <uring|pwrite>   msosm_save_simulated.cpp:124-149
+//   uneven <bw> <f0> <push_blocks> <dms.txt> <in.c64> <out.c64>   per-DM lengths (check_msosm.cpp pattern)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -14,6 +15,7 @@
 
 #include "fold_gpu.h"
 #include "msosm_data_save_uring.h"
+#include "msosm_gpu_uneven.h"
 #include "msosm_gpu_batch.h"
 #include "osm_gpu_batch.h"
 
@@ -52,6 +54,28 @@ int main(int argc, char **argv)
         std::ofstream of(argv[7], std::ios::binary);
         of.write(reinterpret_cast<const char *>(output.data()), output.size() * sizeof(uint16_pair));
         std::cout << "M " << msosm.M << " delaycount " << msosm.delaycount << " calls " << count << std::endl;
+    } else if (mode == "uneven") {
+        std::vector<float> dms;
+        std::ifstream df(argv[5]);
+        for (float v; df >> v;) dms.push_back(v);
+        std::ifstream inf(argv[6], std::ios::binary | std::ios::ate);
+        const size_t n = (size_t)inf.tellg() / sizeof(Complex);
+        inf.seekg(0);
+        std::vector<Complex> input(n);
+        inf.read(reinterpret_cast<char *>(input.data()), n * sizeof(Complex));
+        MSOSM_GPU_UNEVEN msosm(bw, dms.data(), f0, (int)dms.size());
+        msosm.initialize(batch, MSOSM_IN_C64, MSOSM_OUT_C64);
+        const size_t per = (size_t)msosm.push_samples();
+        const size_t count = n / per;
+        std::vector<Complex> output(dms.size() * count * per);
+        for (size_t k = 0; k < count; k++) {
+            msosm.filter_block_c64(input.data() + k * per);
+            for (size_t i = 0; i < dms.size(); i++) msosm.get_output((int)i, output.data() + (i * count + k) * per);
+        }
+        msosm.synchronize();
+        std::ofstream of(argv[7], std::ios::binary);
+        of.write(reinterpret_cast<const char *>(output.data()), output.size() * sizeof(Complex));
+        std::cout << "classes " << msosm.num_classes() << " push " << per << " calls " << count << std::endl;
     } else if (mode == "uring") {
         std::vector<float> dms;
         std::ifstream df(argv[5]);

@@ -110,3 +110,26 @@ def test_data_save_uring_files(tmp_path, writer):
         got = np.fromfile(path, dtype=np.uint32)
         print(f"  {os.path.basename(path)}: {got.size} samples", flush=True)
         assert np.array_equal(got, ref[i])
+
+
+def test_uneven_class_matches_per_dm_oracle(tmp_path):
+    # MSOSM_GPU_UNEVEN (C++): every DM at its own length, driven like check_msosm.cpp
+    dms = [5.0, 150.0, 600.0]
+    design = ms.UnevenDesign(BW, F0, dms)
+    push = 2 * int(design.M.max())
+    calls = 3
+    n = push * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 150.0, 0.0011, seed=25)
+    raw, x = raw[:n], x[:n]
+    (tmp_path / "dms.txt").write_text("\n".join(str(d) for d in dms))
+    raw.astype(np.complex64).tofile(tmp_path / "in.c64")
+    _run(["uneven", BW, F0, 2, "dms.txt", "in.c64", "out.c64"], tmp_path)
+    got = np.fromfile(tmp_path / "out.c64", dtype=np.complex64).reshape(len(dms), n)
+    for i, dm in enumerate(dms):
+        M = int(design.M[i])
+        d, _ = orc.segmentation(BW, F0, float(np.float32(dm)), M)
+        H = orc.transfer_function(BW, F0, float(np.float32(dm)), M)
+        ref = orc.msosm(x, M, d[None], H[None])[0]
+        err = rel_l2(got[i], ref)
+        print(f"  DM {dm} (M={M}): rel-L2 {err:.2e}", flush=True)
+        assert err < 1e-5
