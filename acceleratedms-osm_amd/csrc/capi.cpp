@@ -212,6 +212,18 @@ extern "C" int msosm_fold_accumulate(msosm_fold *f, const void *pol1, const void
     return MSOSM_OK;
 }
 
+extern "C" int msosm_stream_wait(void *waiting, void *producer)
+{
+    if (waiting == producer) return MSOSM_OK;
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev, static_cast<hipStream_t>(producer));
+    if (e == hipSuccess) e = hipStreamWaitEvent(static_cast<hipStream_t>(waiting), ev, 0);
+    if (ev) (void)hipEventDestroy(ev); // the wait keeps what it needs
+    if (e != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string("stream_wait: ") + hipGetErrorString(e));
+    return MSOSM_OK;
+}
+
 extern "C" int msosm_fold_read(msosm_fold *f, float *profile, double *sums, unsigned long long *counts)
 {
     if (!f) return set_error(MSOSM_ERR_ARG, "fold_read: null fold");

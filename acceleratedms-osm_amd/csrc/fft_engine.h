@@ -154,6 +154,19 @@ __device__ __forceinline__ int lds_index(int e, int col)
         return col * (NS + NS / 16) + lds_pad(e);
 }
 
+// lds_index of element A + C where pad(A + C) = pad(A) + C + C/16, i.e. where A + C has no carry out
+// of the low 4 bits of A (C a multiple of 16, or A a multiple of 16). With C known at compile time
+// the per-thread part pad(A) is computed once and each access is an immediate offset from it.
+template <int NS, int NCOL, bool COLFAST>
+__device__ __forceinline__ int lds_index_split(int padA, int C, int col)
+{
+    const int e = padA + C + (C >> 4);
+    if constexpr (COLFAST)
+        return e * NCOL + col;
+    else
+        return col * (NS + NS / 16) + e;
+}
+
 template <int NS, int NCOL, bool COLFAST>
 struct TileLDS {
     static constexpr int ELEMS = (NS + NS / 16) * NCOL;
@@ -221,17 +234,44 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
     constexpr int Ns = Ns_done * R_done; // span at the start of the next pass
     if constexpr (Ns < NS) {
         constexpr int R = PassRadix<NS, Ns>::R;
+        constexpr int TPC = NS / 16;
         __syncthreads(); // LDS free (previous readers done)
+        // output position (j - k) R + k + t Ns, j = jt + q TPC, k = j mod Ns: split into a per-thread
+        // part and a compile-time part without carries when k does not depend on q
+        constexpr bool out_split = (Ns_done == 1 && R_done == 16) || (Ns_done % 16 == 0 && TPC % Ns_done == 0);
+        if constexpr (out_split) {
+            const int kk = jt & (Ns_done - 1);
+            const int padA = lds_pad((jt - kk) * R_done + kk);
 #pragma unroll
-        for (int q = 0; q < 16 / R_done; ++q)
+            for (int q = 0; q < 16 / R_done; ++q)
 #pragma unroll
-            for (int t = 0; t < R_done; ++t)
-                lds[lds_index<NS, NCOL, COLFAST>(out_pos<NS, R_done, Ns_done>(jt, q, t), col)] = v[q * R_done + t];
+                for (int t = 0; t < R_done; ++t)
+                    lds[lds_index_split<NS, NCOL, COLFAST>(padA, q * TPC * R_done + t * Ns_done, col)] =
+                        v[q * R_done + t];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16 / R_done; ++q)
+#pragma unroll
+                for (int t = 0; t < R_done; ++t)
+                    lds[lds_index<NS, NCOL, COLFAST>(out_pos<NS, R_done, Ns_done>(jt, q, t), col)] = v[q * R_done + t];
+        }
         __syncthreads();
+        // input position jt + q TPC + t NS/R
+        constexpr bool in_split = TPC % 16 == 0 && (NS / R) % 16 == 0;
+        if constexpr (in_split) {
+            const int padA = lds_pad(jt);
 #pragma unroll
-        for (int q = 0; q < 16 / R; ++q)
+            for (int q = 0; q < 16 / R; ++q)
 #pragma unroll
-            for (int t = 0; t < R; ++t) v[q * R + t] = lds[lds_index<NS, NCOL, COLFAST>(in_pos<NS, R>(jt, q, t), col)];
+                for (int t = 0; t < R; ++t)
+                    v[q * R + t] = lds[lds_index_split<NS, NCOL, COLFAST>(padA, q * TPC + t * (NS / R), col)];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16 / R; ++q)
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+                    v[q * R + t] = lds[lds_index<NS, NCOL, COLFAST>(in_pos<NS, R>(jt, q, t), col)];
+        }
         butterflies<NS, R, Ns, DIR>(v, jt, tw);
         stockham_rest<NS, NCOL, COLFAST, DIR, R, Ns>(v, lds, col, jt, tw);
     }

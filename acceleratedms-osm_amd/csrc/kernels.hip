@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
             for (int i = 0; i < 16; ++i) {
                 const int f = lt + 256 * i;
                 const int n1 = f / NC, row = f % NC;
-                st32(dst, (uint32_t)(tile * NC + row + N2 * n1), lds[lds_pad(f)]);
+                st32(dst, (uint32_t)(tile * NC + row + N2 * n1), lds[lds_pad(lt) + 272 * i]); // pad(lt + 256 i)
             }
         };
         pass_b<N1, N2, -1>(scr, emit, lds_all, tw);

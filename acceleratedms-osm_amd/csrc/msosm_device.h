@@ -220,7 +220,20 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
 #pragma unroll
             for (int q = 0; q < 16 / R; ++q)
 #pragma unroll
-                for (int t = 0; t < R; ++t) lds[lds_pad(final_pos<NS>(jt, q, t) * NC + row)] = v[q * R + t];
+                for (int t = 0; t < R; ++t) {
+                    // final_pos = A + (q TPC R + t Ns_last) with A per thread (no carries, as in
+                    // stockham_rest); times NC (a multiple of 16) plus row keeps pad linear in it
+                    constexpr int NsL = LastPass<NS>::Ns_last;
+                    constexpr bool split = NC % 16 == 0 && ((NsL == 1 && R == 16) || (NsL % 16 == 0 && TPC % NsL == 0));
+                    if constexpr (split) {
+                        const int kk = jt & (NsL - 1);
+                        const int A = (jt - kk) * R + kk;
+                        const int C = q * TPC * R + t * NsL;
+                        lds[A * (NC + NC / 16) + lds_pad(row) + C * (NC + NC / 16)] = v[q * R + t];
+                    } else {
+                        lds[lds_pad(final_pos<NS>(jt, q, t) * NC + row)] = v[q * R + t];
+                    }
+                }
         }
         __syncthreads();
         emit(lds, tile, active);
@@ -362,7 +375,8 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
             const int f = fbase + i;
             const int n1 = f / NC, row = f % NC;
             const int m = tile * NC + row + N2 * n1 - M;
-            float2 y = lds[lds_pad(f)];
+            // fbase is a multiple of 8 and i < 8: no carry out of the low 4 bits, pad(f) = pad(fbase) + i
+            float2 y = lds[lds_pad(fbase) + i];
             if (oc) st32(oc, (uint32_t)m, y);
             if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
             if constexpr (f16)

@@ -23,6 +23,7 @@ void Fold_GPU::calculate_intensity(T *pol1, T *pol2)
     pol1_d = pol1->get_output_pointer();
     pol2_d = pol2->get_output_pointer();
     stream = pol1->get_stream();
+    stream2 = pol2->get_stream();
 }
 
 template <typename T>
@@ -32,6 +33,7 @@ void Fold_GPU::calculate_intensity(T *pol1)
     pol1_d = pol1->get_output_pointer();
     pol2_d = nullptr;
     stream = pol1->get_stream();
+    stream2 = stream;
 }
 
 template void Fold_GPU::calculate_intensity<MSOSM_GPU_BATCH>(MSOSM_GPU_BATCH *, MSOSM_GPU_BATCH *);
@@ -43,6 +45,8 @@ void Fold_GPU::fold_data_phase()
 {
     // fold_gpu.cpp:48-54 (t_start advances by size * dt inside the fold object)
     if (!pol1_d) msosm::check(MSOSM_ERR_STATE, "fold_data_phase: call calculate_intensity first");
+    // each object dedisperses on its own stream: the fold (on pol1's) waits for pol2's output too
+    if (pol2_d) msosm::check(msosm_stream_wait(stream, stream2), "fold_data_phase");
     msosm::check(msosm_fold_accumulate(fold_, pol1_d, pol2_d, series_stride, stream), "fold_data_phase");
 }
 

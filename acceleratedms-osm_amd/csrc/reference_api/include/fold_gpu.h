@@ -35,7 +35,7 @@ private:
     long long series_stride = 0;
     const void *pol1_d = nullptr;
     const void *pol2_d = nullptr;
-    void *stream = nullptr;
+    void *stream = nullptr, *stream2 = nullptr;
     msosm_fold *fold_ = nullptr;
     ofstream outfile;
 };

@@ -42,7 +42,7 @@ EXPORTS = [
     "msosm_multi_set_graph",
     "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
-    "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune",
+    "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune", "msosm_stream_wait",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
@@ -152,6 +152,7 @@ def lib():
         "msosm_plan_set_schedule": ([vp, C.c_int, C.c_int], C.c_int),
         "msosm_plan_get_schedule": ([vp, ip, ip], C.c_int),
         "msosm_plan_autotune": ([vp, vp, C.c_int, ip, ip, dp], C.c_int),
+        "msosm_stream_wait": ([vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -187,6 +187,9 @@ int msosm_fold_destroy(msosm_fold *fold);
 /* calculate_intensity + fold_data_phase (fold_gpu.cpp:27-54): series s at pol1_dev + s*series_stride
  * complex elements; pol2_dev != NULL selects the two-polarisation amplitude sqrt(|a|^2+|b|^2),
  * otherwise the power |a|^2. stream: hipStream_t or NULL (legacy default stream). */
+/* work queued on waiting_stream after this call starts after the work queued so far on producer_stream
+ * (an event dependency; the reference ran both polarisations on the default stream instead) */
+int msosm_stream_wait(void *waiting_stream, void *producer_stream);
 int msosm_fold_accumulate(msosm_fold *fold, const void *pol1_dev, const void *pol2_dev, long long series_stride,
                           void *stream);
 /* get_folded_data (fold_gpu.cpp:56-60): profile[num_series][nbins] = sum/count (0 where empty);

@@ -6,6 +6,8 @@
 //   osm   <bw> <f0> <batch> <dm> <in.u16> <out.c64>            check_osm.cpp:66-67,127-133
 //   uring <bw> <f0> <batch> <dms.txt> <in.u16> <dir> <uring|pwrite>   msosm_save_simulated.cpp:124-149
 //   uneven <bw> <f0> <push_blocks> <dms.txt> <in.c64> <out.c64>   per-DM lengths (check_msosm.cpp pattern)
+//   dada_info <file.dada> <n>                                   PSRDADA header and first samples (no GPU)
+//   psrdada <file.dada> <dm> <period> <count> <nbins>           msosm_psrdada.cpp:88-153 (dual-pol fold)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -16,6 +18,7 @@
 #include "fold_gpu.h"
 #include "msosm_data_save_uring.h"
 #include "msosm_gpu_uneven.h"
+#include "psrdada.h"
 #include "msosm_gpu_batch.h"
 #include "osm_gpu_batch.h"
 
@@ -33,6 +36,59 @@ int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
     const std::string mode = argv[1];
+    if (mode == "dada_info") {
+        PSRDADA file(argv[2]);
+        file.initBuffer(std::stoul(argv[3]));
+        file.readSamples();
+        std::cout << "bw " << file.bw << " fc " << file.fc << " nbit " << file.nbit << " nchan " << file.nchan
+                  << " ndim " << file.ndim << " npol " << file.npol << " readCount " << file.readCount << " out "
+                  << file.outfileName << std::endl;
+        for (int i = 0; i < 4; i++)
+            std::cout << "s" << i << " " << file.pol1_in[i].first << " " << file.pol1_in[i].second << " "
+                      << file.pol2_in[i].first << " " << file.pol2_in[i].second << std::endl;
+        std::cout << "last " << file.pol1_in[2048].first << " " << file.pol2_in[2048].first << std::endl;
+        return 0;
+    }
+    if (mode == "psrdada") {
+        // run_psrdada_GPU_batch (msosm_psrdada.cpp:88-153): both polarisations dedispersed, amplitude
+        // sqrt(|a|^2 + |b|^2) folded (one polarisation: power)
+        const float dm = std::stof(argv[3]), period = std::stof(argv[4]);
+        const int count = std::stoi(argv[5]);
+        const unsigned long time_bin = std::stoul(argv[6]);
+        PSRDADA file(argv[2]);
+        const float bw = file.bw * 1e6f, f0 = (file.fc - file.bw / 2) * 1e6f;
+        MSOSM_GPU_BATCH gpu1(bw, dm, f0);
+        MSOSM_GPU_BATCH *gpu2 = file.npol == 2 ? new MSOSM_GPU_BATCH(bw, dm, f0) : nullptr;
+        gpu1.initialize_uint16(0, count, true);
+        if (gpu2) gpu2->initialize_uint16(0, count, true);
+        const unsigned long M = gpu1.M;
+        file.initBuffer(count * M);
+        Fold_GPU fold(period, bw, count * M, file.outfileName, time_bin);
+        bool first = true;
+        unsigned long used = 0;
+        for (unsigned long i = 0; i < file.readCount; i++) {
+            file.readSamples();
+            if (first) { // leading invalid data (code 0) is skipped (msosm_psrdada.cpp:131-137)
+                if (file.pol1_in[0].first == 0) continue;
+                first = false;
+            }
+            gpu1.filter_block_uint16(file.pol1_in);
+            if (gpu2) {
+                gpu2->filter_block_uint16(file.pol2_in);
+                fold.calculate_intensity<MSOSM_GPU_BATCH>(&gpu1, gpu2);
+            } else {
+                fold.calculate_intensity<MSOSM_GPU_BATCH>(&gpu1);
+            }
+            fold.fold_data_phase();
+            used++;
+        }
+        fold.get_folded_data();
+        fold.write_to_file();
+        delete gpu2;
+        std::cout << "M " << M << " calls " << used << " of " << file.readCount << " out " << file.outfileName
+                  << std::endl;
+        return 0;
+    }
     const float bw = std::stof(argv[2]), f0 = std::stof(argv[3]);
     const int batch = std::stoi(argv[4]);
     if (mode == "save") {

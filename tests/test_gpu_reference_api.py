@@ -133,3 +133,45 @@ def test_uneven_class_matches_per_dm_oracle(tmp_path):
         err = rel_l2(got[i], ref)
         print(f"  DM {dm} (M={M}): rel-L2 {err:.2e}", flush=True)
         assert err < 1e-5
+
+
+@pytest.mark.parametrize("npol", [2, 1])
+def test_psrdada_dual_pol_fold(tmp_path, npol):
+    # run_psrdada_GPU_batch (msosm_psrdada.cpp:88-153) on a synthetic DADA file: a leading read of
+    # code-0 samples is skipped, each polarisation is dedispersed, and sqrt(|a|^2 + |b|^2) (one
+    # polarisation: |a|^2) is folded (gpu_kernel.cu:430-444, fold_gpu.cpp:48-54)
+    from tests.util import write_dada
+
+    dm, period, count, nbins = 30.0, 0.0011, 4, 64
+    bw_mhz, fc_mhz = 16.0, 1400.0
+    f0 = (fc_mhz - bw_mhz / 2) * 1e6
+    M = ms.Design(bw_mhz * 1e6, f0, [dm]).M
+    per = count * M
+    calls = 3
+    n2 = 1 << (per * calls - 1).bit_length()
+    raw1, x1 = make_input(ms.IN_U16, n2, bw_mhz * 1e6, f0, dm, period, seed=26)
+    raw2, x2 = make_input(ms.IN_U16, n2, bw_mhz * 1e6, f0, dm, period, seed=27)
+    raw1, x1, raw2, x2 = raw1[:per * calls], x1[:per * calls], raw2[:per * calls], x2[:per * calls]
+    lead = np.zeros(per, dtype=np.uint32)
+    path = tmp_path / "psr.dada"
+    write_dada(path, np.concatenate([lead, raw1]), np.concatenate([lead, raw2]) if npol == 2 else None,
+               bw_mhz, fc_mhz)
+    out = _run(["psrdada", str(path), dm, period, count, nbins], tmp_path)
+    assert f"calls {calls} of {calls + 1}" in out
+    prof = np.loadtxt(tmp_path / "psr_dedisp_folded.txt")
+    delays, H = oracle_design(bw_mhz * 1e6, f0, [dm], M)
+    y1 = orc.msosm(x1, M, delays, H)[0]
+    I = np.abs(y1) ** 2
+    if npol == 2:
+        y2 = orc.msosm(x2, M, delays, H)[0]
+        I = np.sqrt(np.abs(y1) ** 2 + np.abs(y2) ** 2)
+    sums, cnts = np.zeros(nbins), np.zeros(nbins, dtype=np.uint64)
+    dt = 1.0 / float(np.float32(bw_mhz * 1e6))
+    t0 = 0.0
+    for c in range(calls):
+        orc.fold(I[c * per:(c + 1) * per], nbins, t0, dt, float(np.float32(period)), sums, cnts)
+        t0 += float(per) * dt
+    ref = np.where(cnts > 0, sums / np.maximum(cnts, 1), 0.0)
+    err = np.max(np.abs(prof - ref) / np.abs(ref))
+    print(f"  npol {npol}: profile max rel err {err:.2e}", flush=True)
+    assert err < 1e-5

@@ -71,3 +71,17 @@ def check_u16_codes(got: np.ndarray, ref: np.ndarray, tie_window: float = 0.02):
         ntot += d.size
     assert worst <= 1
     return ndiff / ntot, worst
+
+
+def write_dada(path, pol1: np.ndarray, pol2: np.ndarray | None, bw_mhz: float, fc_mhz: float):
+    """A DADA file as psrdada/psrdada.cpp reads it: a 4096-byte ASCII header, then frames of 2048
+    uint16-pair samples, one frame per polarisation in turn."""
+    npol = 1 if pol2 is None else 2
+    hdr = (f"HDR_VERSION 1.0\nHDR_SIZE    4096\nBW          {bw_mhz}\nFREQ        {fc_mhz}\nNBIT        16\n"
+           f"NCHAN       1\nNDIM        2\nNPOL        {npol}\n").encode()
+    frames = pol1.reshape(-1, 2048).astype(np.uint32)
+    if npol == 2:
+        frames = np.stack([frames, pol2.reshape(-1, 2048).astype(np.uint32)], axis=1).reshape(-1, 2048)
+    with open(path, "wb") as fh:
+        fh.write(hdr.ljust(4096, b"\0"))
+        fh.write(frames.tobytes())
