@@ -22,6 +22,7 @@ for p in (ROOT, os.path.join(ROOT, "acceleratedms-osm_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+ORDERS = ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8"]
 
 # Uneven grid (--workload uneven): complex64 baseband, 2^26 samples at 32 MS/s, band 784-816 MHz,
 # 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
@@ -76,6 +77,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
     ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
+    ap.add_argument("--order", choices=ORDERS, default=None,
+                    help="fixed unit order instead of the autotuner (profiling runs)")
     ap.add_argument("--refine-steps", type=int, default=40)
     ap.add_argument("--push-blocks", type=int, default=0,
                     help="uneven workload: blocks of the longest length per push (default 4)")
@@ -137,11 +140,15 @@ def main():
     info = plan.info()
     log(f"plan: FBS={info.FBS} workgroups={info.workgroups} device bytes={info.device_bytes / 2**30:.2f} GiB")
     base = data.data_ptr()
-    # tile schedule: measured on the first pushes of the record (this resets the plan and the fold)
-    wg, order, tms = plan.autotune(base, 3)
-    schedule = {"workgroups": wg,
-                "order": ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8"][order]}
-    log(f"autotune: {schedule} ({tms:.3f} ms per dedispersion launch)")
+    if args.order:
+        plan.set_schedule(0, ORDERS.index(args.order))
+        wg, order = plan.schedule()
+        schedule = {"workgroups": wg, "order": ORDERS[order], "autotuned": False}
+    else:
+        # tile schedule: measured on the first pushes of the record (this resets the plan and the fold)
+        wg, order, tms = plan.autotune(base, 3)
+        schedule = {"workgroups": wg, "order": ORDERS[order], "autotuned": True}
+        log(f"autotune: {schedule} ({tms:.3f} ms per dedispersion launch)")
 
     def step():
         for c in range(pushes):
