@@ -389,6 +389,8 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     split_length(log2L, &n1, &n2);
     p->tw_s = (log2L + 1) / 2;
     const int nlo = 1 << p->tw_s, nhi = (int)(L >> p->tw_s);
+    // the dedispersion kernel keeps lo and hi in LDS with the sizes of n1 and n2
+    if (nlo != n1 || nhi != n2) return fail(MSOSM_ERR_ARG, "plan_create: twiddle split does not match the length split");
     p->tw_n1 = 0;
     p->tw_n2 = n1;
     p->tw_lo = n1 + n2;

@@ -31,7 +31,8 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
         __syncthreads();
         const long long slot = (a.g0 + b) % a.FBS;
         float2 *dst = a.ring + (size_t)slot * L;
-        auto emit = [&](const float2 *lds, int tile, bool active) {
+        auto prep = [](int, bool) { return 0; };
+        auto emit = [&](const float2 *lds, int tile, bool active, int) {
             if (!active) return;
             constexpr int NC = S::NCB;
             const int lt = threadIdx.x & 255;
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
                 st32(dst, (uint32_t)(tile * NC + row + N2 * n1), lds[lds_pad(lt) + 272 * i]); // pad(lt + 256 i)
             }
         };
-        pass_b<N1, N2, -1>(scr, emit, lds_all, tw);
+        pass_b<N1, N2, -1>(scr, prep, emit, lds_all, tw);
         __syncthreads();
     }
 }
@@ -54,8 +55,10 @@ template <int N1, int N2, bool RING32, int VAR>
 __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
 {
     using S = Shape<N1, N2>;
-    // tile prefetch where its 32 registers fit without spills (plain epilogue, L <= 2^16)
+    // scratch prefetch where its registers fit without spills (plain epilogue, L <= 2^16); delay
+    // prefetch for every plain length
     constexpr bool PREFETCH = VAR == VAR_PLAIN && S::L <= (1 << 16);
+    constexpr bool PREFETCH_DELAYS = VAR == VAR_PLAIN;
     extern __shared__ float2 lds_all[];
     Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
     load_twiddles<N1, N2>(lds_all, tw);
@@ -96,7 +99,8 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
             b = (int)(s % a.B);
         }
         if (d < a.D && b < a.B) // uniform over the workgroup
-            inv_unit<N1, N2, RING32, VAR, PREFETCH>(a, d, b, lds_all, tw, scr, hsum, hcnt, pw_lds);
+            inv_unit<N1, N2, RING32, VAR, PREFETCH, SUBT, PREFETCH_DELAYS>(a, d, b, lds_all, tw, scr, hsum, hcnt,
+                                                                           pw_lds);
     }
 }
 
@@ -229,6 +233,7 @@ static int lds_bytes_t()
 {
     return Shape<N1, N2>::LDS_BYTES;
 }
+
 
 #ifndef MSOSM_SHAPES
 #define MSOSM_SHAPES(X)                                                                                                \

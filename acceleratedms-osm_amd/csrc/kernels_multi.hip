@@ -11,10 +11,11 @@ namespace msosm {
     X(14, 128, 128) X(15, 256, 128) X(16, 256, 256) X(17, 512, 256) X(18, 512, 512) X(19, 1024, 512)                  \
     X(20, 1024, 1024) X(21, 2048, 1024)
 
+// the multi-length kernel reads every class's twiddle tables from global memory: tiles only
 static constexpr int multi_tiles_bytes()
 {
     int m = 0;
-#define X(lg, a, b) m = m > Shape<a, b>::LDS_BYTES ? m : Shape<a, b>::LDS_BYTES;
+#define X(lg, a, b) m = m > Shape<a, b>::LDS_TILES_BYTES ? m : Shape<a, b>::LDS_TILES_BYTES;
     MSOSM_MULTI_SHAPES(X)
 #undef X
     return m;
@@ -50,7 +51,10 @@ __global__ void __launch_bounds__(WG) inv_multi_kernel(MultiInvArgs a)
         const Tw tw{ca.tw_n1, ca.tw_n2, ca.tw_lo, ca.tw_hi, ca.tw_s};
         switch (a.log2L[c]) {
 #define X(lg, n1, n2)                                                                                                  \
-    case lg: inv_unit<n1, n2, true, VAR>(ca, d, b, lds_all, tw, scr, hsum, hcnt, pw_lds); break;
+    case lg:                                                                                                           \
+        inv_unit<n1, n2, true, VAR, VAR == VAR_PLAIN && lg <= 16, SUBT, VAR == VAR_PLAIN>(ca, d, b, lds_all, tw, scr,  \
+                                                                                         hsum, hcnt, pw_lds);          \
+        break;
             MSOSM_MULTI_SHAPES(X)
 #undef X
         }

@@ -32,7 +32,7 @@ struct TileWords {
     static constexpr int V = AB > T ? AB : T;
 };
 
-template <int N1, int N2>
+template <int N1, int N2, int NSUB = SUBT>
 struct Shape {
     static constexpr int L = N1 * N2;
     static constexpr int NCA = TILE / N2 < 1 ? 1 : TILE / N2; // columns per pass-A tile
@@ -41,11 +41,14 @@ struct Shape {
     static constexpr int NTB = N2 / NCB;                      // pass-B tiles
     static constexpr int WA = TileWords<N2>::V, WB = TileWords<N1>::V;
     static constexpr int TILE_WORDS = WA > WB ? WA : WB;
-    static constexpr int LDS_TILES_BYTES = SUBT * TILE_WORDS * 8;
-    // sub-FFT twiddle tables for N1 and N2 follow the tiles when they are small (<= 8 KiB; the
-    // largest lengths read them from global memory); the fold histogram comes last
-    static constexpr bool TW_IN_LDS = false; // measured: global (L1-resident) tables are faster here
-    static constexpr int LDS_TW_BYTES = TW_IN_LDS ? (N1 + N2) * 8 : 0;
+    static constexpr int LDS_TILES_BYTES = NSUB * TILE_WORDS * 8;
+    // The twiddle tables (sub-FFT N1 and N2, four-step lo and hi: 2 (N1 + N2) entries) follow the
+    // tiles in LDS where they fit beside a 1024-bin fold histogram: their reads then count on the LDS
+    // counter and never wait for the prefetched next tile's global loads. Otherwise they stay in
+    // global memory (L1-resident). The fold histogram comes last.
+    static constexpr int TW_BYTES = 2 * (N1 + N2) * 8;
+    static constexpr bool TW_IN_LDS = NSUB * TILE_WORDS * 8 + TW_BYTES + 1024 * 12 + 16 <= 160 * 1024;
+    static constexpr int LDS_TW_BYTES = TW_IN_LDS ? TW_BYTES : 0;
     static constexpr int LDS_BYTES = LDS_TILES_BYTES + LDS_TW_BYTES;
 };
 
@@ -73,19 +76,32 @@ struct Tw {
     int s;
 };
 
-// copy the N1- and N2-point twiddle tables into LDS (read once per kernel)
-template <int N1, int N2>
+// copy the twiddle tables into LDS (read once per kernel) where the shape keeps them there. The
+// four-step tables are the split lo = e^{2 pi i m/L} (m < 2^s), hi = e^{2 pi i m 2^s/L} with
+// 2^s = N1, the split the host builds for every length (checked at plan creation). All four
+// pointers then refer to LDS only, so their reads compile to LDS instructions (a pointer that may
+// refer to either memory compiles to flat loads, which wait on both memory counters).
+template <int N1, int N2, int NSUB = SUBT>
 __device__ __forceinline__ void load_twiddles(float2 *lds_all, Tw &tw)
 {
-    using S = Shape<N1, N2>;
-    if constexpr (!S::TW_IN_LDS) return;
-    float2 *t1 = lds_all + S::LDS_TILES_BYTES / 8;
-    float2 *t2 = t1 + N1;
-    for (int i = threadIdx.x; i < N1; i += blockDim.x) t1[i] = tw.n1[i];
-    for (int i = threadIdx.x; i < N2; i += blockDim.x) t2[i] = tw.n2[i];
-    tw.n1 = t1;
-    tw.n2 = t2;
-    __syncthreads();
+    using S = Shape<N1, N2, NSUB>;
+    if constexpr (S::TW_IN_LDS) {
+        float2 *t1 = lds_all + S::LDS_TILES_BYTES / 8;
+        float2 *t2 = t1 + N1, *tl = t2 + N2, *th = tl + N1;
+        for (int i = threadIdx.x; i < N1; i += blockDim.x) {
+            t1[i] = tw.n1[i];
+            tl[i] = tw.lo[i];
+        }
+        for (int i = threadIdx.x; i < N2; i += blockDim.x) {
+            t2[i] = tw.n2[i];
+            th[i] = tw.hi[i];
+        }
+        tw.n1 = t1;
+        tw.n2 = t2;
+        tw.lo = tl;
+        tw.hi = th;
+        __syncthreads();
+    }
 }
 
 template <int DIR>
@@ -99,12 +115,58 @@ __device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
 // Pass A: column sub-FFTs. load(k) returns input element k (k = k1 + N1*k2).
 // Output Z[k1][n2] * W^{DIR n2 k1} goes to scr[(tile * N2 + n2) * NCA + col].
 // ---------------------------------------------------------------------------------------------
+// Materialises v here: keeps the compiler from sinking the computation of v below the next tile's
+// (conditional) loads, which would keep the consumed registers alive across them (and copy them).
+__device__ __forceinline__ void pin16(float2 (&v)[16])
+{
+#pragma unroll
+    for (int t = 0; t < 16; ++t) asm volatile("" : "+v"(v[t].x), "+v"(v[t].y));
+}
+
+// One pass-A tile step: the column sub-FFTs of the tile this thread's 256-thread group works on,
+// then the W^{n2 k1} twiddle and the store into the scratch slot (inactive groups only take part in
+// the barriers).
+template <int N1, int N2, int DIR, int NSUB>
+__device__ __forceinline__ void pass_a_tile(float2 (&v)[16], float2 *__restrict__ scr, float2 *lds_all, const Tw &tw,
+                                            int t0)
+{
+    using S = Shape<N1, N2, NSUB>;
+    constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
+    // opaque thread index: keeps per-thread addresses from being hoisted (and spilled) across tiles
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int sub = tid >> 8, lt = tid & 255;
+    const int col = lt % NC, jt = lt / NC;
+    float2 *lds = lds_all + sub * S::TILE_WORDS;
+    const int tile = t0 + sub;
+    const bool active = (S::NTA % NSUB == 0) || tile < S::NTA;
+    const int k1 = tile * NC + col;
+    subfft<NS, NC, true, DIR>(v, lds, col, jt, tw.n2);
+    if (active) {
+        // element (q, t) is output n2 = jt + q*TPC + t*NsL; its twiddle W^{n2 k1} is
+        // W^{(jt + q*TPC) k1} (table) times c^t with c = W^{NsL k1} from binary powers, so a
+        // thread does 2 + 2*NB table reads per tile instead of two per element
+        constexpr int R = FinalShape<NS>::R, NsL = NS / R;
+        const float2 c = big_twiddle<DIR>(tw, k1 * NsL);
+#pragma unroll
+        for (int q = 0; q < 16 / R; ++q) {
+            float2 w = big_twiddle<DIR>(tw, (jt + q * TPC) * k1);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const int n2 = final_pos<NS>(jt, q, t);
+                st32(scr, (uint32_t)((tile * N2 + n2) * NC + col), cmul(v[q * R + t], w));
+                if (t + 1 < R) w = cmul(w, c); // depth <= 15 products: < 1e-6 relative
+            }
+        }
+    }
+}
+
 // With PF, the next tile's inputs are loaded while the current tile is transformed (32 more VGPRs):
 // the loads' latency then overlaps the sub-FFT instead of stalling every wave at the tile start.
-template <int N1, int N2, int DIR, bool PF = false, class Load>
+template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, class Load>
 __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ scr, float2 *lds_all, const Tw &tw)
 {
-    using S = Shape<N1, N2>;
+    using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
     auto fetch = [&](int t0, float2 *dst) {
         int tid = threadIdx.x;
@@ -112,7 +174,7 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
         const int sub = tid >> 8, lt = tid & 255;
         const int col = lt % NC, jt = lt / NC;
         const int tile = t0 + sub;
-        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA; // all sub-tiles busy when divisible
+        const bool active = (S::NTA % NSUB == 0) || tile < S::NTA; // all sub-tiles busy when divisible
         const int k1 = tile * NC + col;
         if (active) {
 #pragma unroll
@@ -125,42 +187,73 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
     float2 nx[16];
     if constexpr (PF) fetch(0, nx);
 #pragma unroll 1
-    for (int t0 = 0; t0 < S::NTA; t0 += SUBT) {
+    for (int t0 = 0; t0 < S::NTA; t0 += NSUB) {
         float2 v[16];
         if constexpr (PF) {
 #pragma unroll
             for (int t = 0; t < 16; ++t) v[t] = nx[t];
-            if (t0 + SUBT < S::NTA) fetch(t0 + SUBT, nx);
+            if (t0 + NSUB < S::NTA) fetch(t0 + NSUB, nx);
         } else {
             fetch(t0, v);
         }
-        // opaque thread index: keeps per-thread addresses from being hoisted (and spilled) across tiles
+        pass_a_tile<N1, N2, DIR, NSUB>(v, scr, lds_all, tw, t0);
+    }
+}
+
+// Pass A of the dedispersion unit with the per-bin delays of the next tile in flight (16 VGPRs): at
+// each tile step the ring and chirp loads are issued at once, from delays already in registers, so a
+// step waits for one memory round trip instead of two dependent ones.
+template <int N1, int N2, bool RING32, bool POWER, int NSUB>
+__device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring, int FBS, int gm,
+                                                 const float2 *__restrict__ Hd, const uint16_t *__restrict__ dl,
+                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, float &power)
+{
+    using S = Shape<N1, N2, NSUB>;
+    constexpr int L = S::L, NC = S::NCA, TPC = N2 / 16, NT = S::NTA;
+    constexpr bool FULL = NT % NSUB == 0;
+    constexpr int LAST = (NT - 1) / NSUB * NSUB;
+    int dd[16];
+    auto load_delays = [&](int t0) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int sub = tid >> 8, lt = tid & 255;
-        const int col = lt % NC, jt = lt / NC;
-        float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
-        const bool active = (S::NTA % SUBT == 0) || tile < S::NTA;
-        const int k1 = tile * NC + col;
-        subfft<NS, NC, true, DIR>(v, lds, col, jt, tw.n2);
-        if (active) {
-            // element (q, t) is output n2 = jt + q*TPC + t*NsL; its twiddle W^{n2 k1} is
-            // W^{(jt + q*TPC) k1} (table) times c^t with c = W^{NsL k1} from binary powers, so a
-            // thread does 2 + 2*NB table reads per tile instead of two per element
-            constexpr int R = FinalShape<NS>::R, NsL = NS / R;
-            const float2 c = big_twiddle<DIR>(tw, k1 * NsL);
+        if (FULL || tile < NT) {
+            const uint32_t k0 = (uint32_t)(lt % NC + N1 * (lt / NC) + tile * NC);
 #pragma unroll
-            for (int q = 0; q < 16 / R; ++q) {
-                float2 w = big_twiddle<DIR>(tw, (jt + q * TPC) * k1);
-#pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    const int n2 = final_pos<NS>(jt, q, t);
-                    st32(scr, (uint32_t)((tile * N2 + n2) * NC + col), cmul(v[q * R + t], w));
-                    if (t + 1 < R) w = cmul(w, c); // depth <= 15 products: < 1e-6 relative
-                }
-            }
+            for (int t = 0; t < 16; ++t) dd[t] = (int)ld32(dl, k0 + (uint32_t)(N1 * TPC * t));
         }
+    };
+    load_delays(0);
+#pragma unroll 1
+    for (int t0 = 0; t0 < NT; t0 += NSUB) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int tile = t0 + sub;
+        float2 x[16], h[16];
+        if (FULL || tile < NT) {
+            const uint32_t k0 = (uint32_t)(lt % NC + N1 * (lt / NC) + tile * NC);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const uint32_t k = k0 + (uint32_t)(N1 * TPC * t);
+                int src = gm - dd[t]; // delays are < FBS, so one wrap at most
+                if (src < 0) src += FBS;
+                x[t] = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + k) : ring[(size_t)src * L + k];
+                h[t] = ld32(Hd, k);
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) x[t] = h[t] = make_float2(0.f, 0.f);
+        }
+        load_delays(min(t0 + NSUB, LAST)); // the last step re-reads its own delays (L2)
+        float2 v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            v[t] = cmul(x[t], h[t]);
+            if constexpr (POWER) power += v[t].x * v[t].x + v[t].y * v[t].y;
+        }
+        pass_a_tile<N1, N2, +1, NSUB>(v, scr, lds_all, tw, t0);
     }
 }
 
@@ -170,10 +263,14 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
 // f = n1 * NCB + row, n2 = tile * NCB + row. Epilogues read f = lane + 256 * i, so consecutive
 // lanes hold consecutive samples (runs of NCB) and neither the LDS reads nor the stores conflict.
 // ---------------------------------------------------------------------------------------------
-template <int N1, int N2, int DIR, bool PF = false, class Emit>
-__device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emit &emit, float2 *lds_all, const Tw &tw)
+// prep(tile, active) runs at the start of each tile step, before the next tile's prefetch is issued
+// (global loads the epilogue needs: they retire before the prefetch, so waiting for them does not
+// wait for it); its result is handed to emit(lds, tile, active, prepared).
+template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, class Prep, class Emit>
+__device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Prep &prep, const Emit &emit,
+                                       float2 *lds_all, const Tw &tw)
 {
-    using S = Shape<N1, N2>;
+    using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
     auto fetch = [&](int t0, float2 *dst) {
         int tid = threadIdx.x;
@@ -181,7 +278,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
         const int sub = tid >> 8, lt = tid & 255;
         const int jt = lt % TPC, row = lt / TPC;
         const int tile = t0 + sub;
-        const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
+        const bool active = (S::NTB % NSUB == 0) || tile < S::NTB;
         const int n2 = tile * NC + row;
         if (active) {
 #pragma unroll
@@ -197,23 +294,32 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
     float2 nx[16];
     if constexpr (PF) fetch(0, nx);
 #pragma unroll 1
-    for (int t0 = 0; t0 < S::NTB; t0 += SUBT) {
-        float2 v[16];
-        if constexpr (PF) {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) v[t] = nx[t];
-            if (t0 + SUBT < S::NTB) fetch(t0 + SUBT, nx);
-        } else {
-            fetch(t0, v);
-        }
+    for (int t0 = 0; t0 < S::NTB; t0 += NSUB) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int sub = tid >> 8, lt = tid & 255;
         const int jt = lt % TPC, row = lt / TPC;
         float2 *lds = lds_all + sub * S::TILE_WORDS;
         const int tile = t0 + sub;
-        const bool active = (S::NTB % SUBT == 0) || tile < S::NTB;
-        subfft<NS, NC, false, DIR>(v, lds, row, jt, tw.n1);
+        const bool active = (S::NTB % NSUB == 0) || tile < S::NTB;
+        const auto prepared = prep(tile, active);
+        float2 v[16];
+        if constexpr (PF) {
+            // the first radix-16 butterfly consumes the prefetched values; the next tile's loads
+            // then reuse their registers (issued before the rest of the transform)
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = nx[t];
+            butterflies<NS, 16, 1, DIR>(v, 0, tw.n1);
+            pin16(v);
+            if (t0 + NSUB < S::NTB) fetch(t0 + NSUB, nx);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            fetch(t0, v);
+        }
+        if constexpr (PF)
+            stockham_rest<NS, NC, false, DIR, 16, 1>(v, lds, row, jt, tw.n1);
+        else
+            subfft<NS, NC, false, DIR>(v, lds, row, jt, tw.n1);
         __syncthreads();
         {
             constexpr int R = FinalShape<NS>::R;
@@ -236,7 +342,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Emi
                 }
         }
         __syncthreads();
-        emit(lds, tile, active);
+        emit(lds, tile, active, prepared);
     }
 }
 
@@ -296,11 +402,13 @@ constexpr int VAR_PLAIN = 0, VAR_F16 = 1, VAR_TANGENT = 2;
 // One dedispersion unit (DM d, block b): gather * H -> IDFT_L -> keep [M, L) * 1/L -> outputs and
 // fold. hsum/hcnt are the workgroup's LDS fold histogram (zero on entry and on exit), pw_lds the
 // FP16-scale reduction word (zero on entry and on exit).
-template <int N1, int N2, bool RING32, int VAR, bool PF = false>
+// With PF pass B prefetches its next scratch tile (32 VGPRs); with PFA the gather runs with the
+// next tile's delays in flight (16 VGPRs).
+template <int N1, int N2, bool RING32, int VAR, bool PF = false, int NSUB = SUBT, bool PFA = PF>
 __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
                                          float2 *__restrict__ scr, double *hsum, unsigned int *hcnt, float *pw_lds)
 {
-    using S = Shape<N1, N2>;
+    using S = Shape<N1, N2, NSUB>;
     constexpr int L = S::L, M = L / 2;
     const bool fold = a.fold_sum != nullptr;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
@@ -325,7 +433,10 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
         if constexpr (f16) power += y.x * y.x + y.y * y.y;
         return y;
     };
-    pass_a<N1, N2, +1, PF>(load, scr, lds_all, tw);
+    if constexpr (PFA)
+        pass_a_gather_dl<N1, N2, RING32, f16, NSUB>(ring, FBS, gm, Hd, dl, scr, lds_all, tw, power);
+    else
+        pass_a<N1, N2, +1, PF, NSUB>(load, scr, lds_all, tw);
     float inv_scale = 1.f, out_scale = 1.f;
     if constexpr (f16) {
         // block rms of the series over the L outputs: sqrt(sum |Y|^2) with Y = X H / L (Parseval);
@@ -348,21 +459,29 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
     uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
     const uint16_t *bins = a.bins ? a.bins + (size_t)b * M : nullptr;
     const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * M : nullptr;
-    auto emit = [&](const float2 *lds, int tile, bool active) {
+    // upper half of a tile's outputs (n1 >= N1/2): 8 consecutive f per thread, i.e. runs of
+    // consecutive samples; the fold accumulates a run in registers and flushes on a bin change.
+    // Runs of >= 8 consecutive samples: their 8 bins are one aligned 16-byte load, issued by prep.
+    auto prep = [&](int tile, bool active) -> uint4 {
+        constexpr int NC = S::NCB;
+        uint4 b4 = make_uint4(0u, 0u, 0u, 0u);
+        if (fold && NC >= 8 && active) {
+            const int fbase = (N1 / 2) * NC + (tid & 255) * 8;
+            const int n1 = fbase / NC, row = fbase % NC;
+            b4 = *reinterpret_cast<const uint4 *>(bins + (tile * NC + row + N2 * n1 - M));
+        }
+        return b4;
+    };
+    auto emit = [&](const float2 *lds, int tile, bool active, const uint4 b4) {
         if (!active) return;
         constexpr int NC = S::NCB;
         const int lt = tid & 255;
-        // upper half of the tile's outputs (n1 >= N1/2): 8 consecutive f per thread, i.e. runs
-        // of consecutive samples; the fold accumulates a run in registers and flushes on a bin change
         const int fbase = (N1 / 2) * NC + lt * 8;
         int cur = -1;
         double csum = 0.0;
         unsigned int ccnt = 0;
-        // runs of >= 8 consecutive samples: their 8 bins are one aligned 16-byte load
         uint16_t bv[8];
         if (fold && NC >= 8) {
-            const int n1 = fbase / NC, row = fbase % NC;
-            const uint4 b4 = *reinterpret_cast<const uint4 *>(bins + (tile * NC + row + N2 * n1 - M));
             const uint32_t w4[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -411,13 +530,13 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
             atomicAdd(&hcnt[cur], ccnt);
         }
     };
-    pass_b<N1, N2, +1, PF>(scr, emit, lds_all, tw);
+    pass_b<N1, N2, +1, PF, NSUB>(scr, prep, emit, lds_all, tw);
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f; // next unit's reduction (ordered by the barriers)
     if (fold) {
         double *gs = a.fold_sum + (size_t)d * a.nbins;
         unsigned long long *gc = a.fold_cnt + (size_t)d * a.nbins;
-        for (int i = tid; i < a.nbins; i += WG) {
+        for (int i = tid; i < a.nbins; i += 256 * NSUB) {
             const unsigned int c = hcnt[i];
             if (c) {
                 unsafeAtomicAdd(&gs[i], hsum[i]);
