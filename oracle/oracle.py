@@ -214,3 +214,52 @@ def quantize_int8(x: np.ndarray, rms: float = 32.0) -> np.ndarray:
 def int8_to_complex(q: np.ndarray) -> np.ndarray:
     q = q.astype(np.float64)
     return q[0::2] + 1j * q[1::2]
+
+
+# ---- FP32 CPU port (the timed CPU baseline; oracle/cpu_msosm.c) ----
+_CPU_PATH = os.path.join(_HERE, "_build", "liboracle_cpu.so")
+_cpu = None
+
+
+def cpu_lib():
+    global _cpu
+    if _cpu is None:
+        if not os.path.exists(_CPU_PATH):
+            build()
+        L = C.CDLL(_CPU_PATH)
+        L.cpu_msosm_backend.argtypes = [C.c_int]
+        L.cpu_msosm_backend.restype = C.c_char_p
+        L.cpu_msosm_fold.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_ulong,
+                                     C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_int]
+        L.cpu_msosm_fold.restype = C.c_int
+        _cpu = L
+    return _cpu
+
+
+def cpu_backend(force_own: bool = False) -> str:
+    """Name of the transform backend of the CPU port (MKL's FFTW3 interface, or its own FFT)."""
+    return cpu_lib().cpu_msosm_backend(int(force_own)).decode()
+
+
+def cpu_msosm_fold(x: np.ndarray, M: int, delays: np.ndarray, H: np.ndarray, nbins: int, t_start: float, dt: float,
+                   period: float, nthreads: int = 0, force_own: bool = False, want_out: bool = False):
+    """FP32 MS-OSM + fold of every DM on the CPU (OpenMP over DMs). x: complex baseband (any complex
+    dtype, cast to complex64), delays: [D][2M] block delays, H: [D][2M] transfer functions (without
+    the 1/(2M) of the inverse transform). Returns (sums [D][nbins], counts [D][nbins], series [D][n]
+    complex64 or None)."""
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    delays = np.ascontiguousarray(delays, dtype=np.uint16)
+    H = np.ascontiguousarray(H, dtype=np.complex64)
+    D, L = delays.shape
+    if L != 2 * M or H.shape != (D, L):
+        raise ValueError("delays and H must be [D][2M]")
+    sums = np.zeros((D, nbins))
+    cnts = np.zeros((D, nbins), dtype=np.uint64)
+    out = np.zeros((D, x.size), dtype=np.complex64) if want_out else None
+    rc = cpu_lib().cpu_msosm_fold(x.ctypes.data, x.size, M, D, delays.ctypes.data, H.ctypes.data, nbins, t_start, dt,
+                                  period, sums.ctypes.data, cnts.ctypes.data,
+                                  out.ctypes.data if out is not None else None, nthreads, int(force_own))
+    if rc != 0:
+        raise RuntimeError(f"cpu_msosm_fold failed ({rc})")
+    return sums, cnts, out
