@@ -53,7 +53,7 @@ def host_threads():
     return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
 
 
-def cpu_baseline(design, n_samples=1 << 24):
+def cpu_baseline(design, n_samples=1 << 25):
     """FP32 CPU port of the same MS-OSM + fold (oracle/cpu_msosm.c: FFTW3-API transforms from Intel MKL,
     OpenMP over DM trials on every host thread) on a bounded sample: all 512 trials of the grid over the
     first n_samples samples of a uniform int8 input, dedispersed and folded to the same 1024 bins."""
