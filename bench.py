@@ -68,7 +68,7 @@ def cpu_baseline(design, n_samples=1 << 25):
     W = WORKLOAD
     t0 = time.perf_counter()
     orc.cpu_msosm_fold(x, design.M, design.delays, design.H, W["nbins"], 0.0, 1.0 / W["fs_hz"], W["period_s"],
-                       nthreads=threads)
+                       nthreads=threads, call_size=W["batch"] * design.M)
     dt = time.perf_counter() - t0
     n_dm = len(design.dms)
     return {"value": n_dm * n_samples / dt, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port",

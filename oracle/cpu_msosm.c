@@ -172,9 +172,12 @@ const char *cpu_msosm_backend(int force_own)
     return (B.mkl && !force_own) ? B.name : OWN_NAME;
 }
 
+/* The record is folded as calls of call_size samples (0: one call), as Fold_GPU::fold_data_phase folds
+ * each call's series: call c bins its sample i at t_start_c + i*dt with t_start_{c+1} = t_start_c +
+ * call_size*dt (fold_gpu.cpp:48-54). */
 int cpu_msosm_fold(const float *x, long n, int M, int D, const uint16_t *delays, const float *H, unsigned long nbins,
-                   double t_start, double dt, double period, double *sums, unsigned long long *cnts, float *out,
-                   int nthreads, int force_own)
+                   double t_start, double dt, double period, long call_size, double *sums, unsigned long long *cnts,
+                   float *out, int nthreads, int force_own)
 {
     if (!x || n <= 0 || M <= 0 || (n % M) != 0 || D <= 0 || !delays || !H || nbins == 0 || nbins > 65536 || !sums ||
         !cnts)
@@ -184,6 +187,12 @@ int cpu_msosm_fold(const float *x, long n, int M, int D, const uint16_t *delays,
     const int L = 2 * M;
     const long nblk = n / M;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
+    const long call = call_size > 0 ? call_size : n;
+    const long ncalls = (n + call - 1) / call;
+    double *ts = (double *)malloc(sizeof(double) * (size_t)ncalls);
+    if (!ts) return -1;
+    ts[0] = t_start;
+    for (long c = 1; c < ncalls; ++c) ts[c] = ts[c - 1] + (double)call * dt;
     float *spec = NULL;
     if (posix_memalign((void **)&spec, 64, sizeof(float) * 2 * (size_t)L * (size_t)nblk)) spec = NULL;
     uint16_t *bins = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)n);
@@ -196,7 +205,7 @@ int cpu_msosm_fold(const float *x, long n, int M, int D, const uint16_t *delays,
         {
             worker *w = &ws[omp_get_thread_num()];
 #pragma omp for schedule(static)
-            for (long i = 0; i < n; ++i) bins[i] = (uint16_t)fold_bin(i, nbins, t_start, dt, period);
+            for (long i = 0; i < n; ++i) bins[i] = (uint16_t)fold_bin(i % call, nbins, ts[i / call], dt, period);
             /* 1. spectra of every block */
 #pragma omp for schedule(static)
             for (long g = 0; g < nblk; ++g) {
@@ -248,5 +257,6 @@ int cpu_msosm_fold(const float *x, long n, int M, int D, const uint16_t *delays,
     free(ws);
     free(bins);
     free(spec);
+    free(ts);
     return rc;
 }

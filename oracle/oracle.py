@@ -230,8 +230,8 @@ def cpu_lib():
         L.cpu_msosm_backend.argtypes = [C.c_int]
         L.cpu_msosm_backend.restype = C.c_char_p
         L.cpu_msosm_fold.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_ulong,
-                                     C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
-                                     C.c_int, C.c_int]
+                                     C.c_double, C.c_double, C.c_double, C.c_long, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int, C.c_int]
         L.cpu_msosm_fold.restype = C.c_int
         _cpu = L
     return _cpu
@@ -243,11 +243,13 @@ def cpu_backend(force_own: bool = False) -> str:
 
 
 def cpu_msosm_fold(x: np.ndarray, M: int, delays: np.ndarray, H: np.ndarray, nbins: int, t_start: float, dt: float,
-                   period: float, nthreads: int = 0, force_own: bool = False, want_out: bool = False):
+                   period: float, nthreads: int = 0, force_own: bool = False, want_out: bool = False,
+                   call_size: int = 0):
     """FP32 MS-OSM + fold of every DM on the CPU (OpenMP over DMs). x: complex baseband (any complex
     dtype, cast to complex64), delays: [D][2M] block delays, H: [D][2M] transfer functions (without
-    the 1/(2M) of the inverse transform). Returns (sums [D][nbins], counts [D][nbins], series [D][n]
-    complex64 or None)."""
+    the 1/(2M) of the inverse transform). The fold bins the record as calls of call_size samples
+    (0: one call), t_start advancing by call_size*dt per call as Fold_GPU does. Returns (sums
+    [D][nbins], counts [D][nbins], series [D][n] complex64 or None)."""
     x = np.ascontiguousarray(x, dtype=np.complex64)
     delays = np.ascontiguousarray(delays, dtype=np.uint16)
     H = np.ascontiguousarray(H, dtype=np.complex64)
@@ -258,7 +260,7 @@ def cpu_msosm_fold(x: np.ndarray, M: int, delays: np.ndarray, H: np.ndarray, nbi
     cnts = np.zeros((D, nbins), dtype=np.uint64)
     out = np.zeros((D, x.size), dtype=np.complex64) if want_out else None
     rc = cpu_lib().cpu_msosm_fold(x.ctypes.data, x.size, M, D, delays.ctypes.data, H.ctypes.data, nbins, t_start, dt,
-                                  period, sums.ctypes.data, cnts.ctypes.data,
+                                  period, call_size, sums.ctypes.data, cnts.ctypes.data,
                                   out.ctypes.data if out is not None else None, nthreads, int(force_own))
     if rc != 0:
         raise RuntimeError(f"cpu_msosm_fold failed ({rc})")

@@ -52,3 +52,19 @@ def test_cpu_port_rejects_bad_shapes(case):
         orc.cpu_msosm_fold(x, M, delays[:, :M], H, NBINS, 0.0, 1.0 / BW, PERIOD)
     with pytest.raises(RuntimeError):
         orc.cpu_msosm_fold(x[: M + 1], M, delays, H, NBINS, 0.0, 1.0 / BW, PERIOD)
+
+
+def test_cpu_port_folds_in_calls_like_fold_gpu(case):
+    # calls of 4M samples: t_start advances by 4M*dt per call (fold_gpu.cpp:53), as the oracle's fold
+    # is driven call by call in the GPU tests
+    x, delays, H, ref = case
+    per = 4 * M
+    sums, cnts, _ = orc.cpu_msosm_fold(x, M, delays, H, NBINS, 0.0, 1.0 / BW, PERIOD, call_size=per)
+    for d in range(len(DMS)):
+        rs, rc = np.zeros(NBINS), np.zeros(NBINS, dtype=np.uint64)
+        t0 = 0.0
+        for c in range(x.size // per):
+            orc.fold(np.abs(ref[d, c * per:(c + 1) * per]) ** 2, NBINS, t0, 1.0 / BW, PERIOD, rs, rc)
+            t0 += float(per) * (1.0 / BW)
+        assert np.array_equal(cnts[d], rc)
+        np.testing.assert_allclose(sums[d], rs, rtol=2e-5)
