@@ -77,6 +77,7 @@ hipError_t launch_inv_multi(const MultiInvArgs &a, int grid, hipStream_t st);
 
 void split_length(int log2L, int *n1, int *n2);
 bool length_supported(int log2L);
+bool split_supported(int log2L); // a role-split dedispersion kernel exists for this length
 int tiles_lds_bytes(int log2L); // tiles and LDS twiddle tables of the forward and dedispersion kernels
 // LDS the inverse kernel needs beyond its tiles and fold histogram (FP16 scale reduction)
 constexpr int INV_EXTRA_LDS = 16;

@@ -67,41 +67,37 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
-    // XCD-aware virtual id: workgroups w, w+8, ... share an XCD; give them consecutive virtual ids, so
-    // that the workgroups of one XCD (and its L2) run consecutive units together
-    const int G = gridDim.x;
-    const int w = blockIdx.x;
-    // unit orders (one loop, so that the unit body is inlined once):
-    //   tiles: the G/8 workgroups of an XCD run dg DMs x (G/8)/dg consecutive blocks at a time, so each
-    //          DM's chirp and delays and each block's ring slots are shared in that XCD's L2;
-    //   XCD DM-/block-major: workgroups w, w+8, ... share an XCD and get consecutive virtual ids;
-    //   linear: workgroup w runs units w, w+G, ...
-    const int dg = a.order == MSOSM_ORDER_XCD_TILE4 ? 4 : a.order == MSOSM_ORDER_XCD_TILE8 ? 8 : 0;
-    const bool tiles = dg && G % 8 == 0 && (G / 8) % dg == 0;
-    const bool xcd = a.order != MSOSM_ORDER_LINEAR && G % 8 == 0;
-    const bool block_major = a.order == MSOSM_ORDER_XCD_BLOCK_MAJOR;
-    const int bg = tiles ? (G / 8) / dg : 1, j = w / 8;
-    const int tb = (a.B + bg - 1) / bg, td = tiles ? (a.D + dg - 1) / dg : 0;
-    const long long first = tiles ? w % 8 : xcd ? (w % 8) * (G / 8) + w / 8 : w;
-    const long long end = tiles ? (long long)td * tb : (long long)a.D * a.B;
-    const long long step = tiles ? 8 : G;
+    const UnitOrder o(a, gridDim.x, blockIdx.x);
 #pragma unroll 1
-    for (long long s = first; s < end; s += step) {
+    for (long long s = o.first; s < o.end; s += o.step) {
         int d, b;
-        if (tiles) {
-            d = (int)(s / tb) * dg + j / bg;
-            b = (int)(s % tb) * bg + j % bg;
-        } else if (block_major) {
-            d = (int)(s % a.D);
-            b = (int)(s / a.D);
-        } else {
-            d = (int)(s / a.B);
-            b = (int)(s % a.B);
-        }
-        if (d < a.D && b < a.B) // uniform over the workgroup
+        if (o.at(s, a, d, b)) // uniform over the workgroup
             inv_unit<N1, N2, RING32, VAR, PREFETCH, SUBT, PREFETCH_DELAYS>(a, d, b, lds_all, tw, scr, hsum, hcnt,
                                                                            pw_lds);
     }
+}
+
+// Role-split form of the plain dedispersion kernel for square lengths (see inv_rs).
+template <int N, bool RING32>
+__global__ void __launch_bounds__(WG) inv_rs_kernel(InvArgs a)
+{
+    using S = Shape<N, N>;
+    extern __shared__ float2 lds_all[];
+    Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    load_twiddles<N, N>(lds_all, tw);
+    float2 *scr = a.scratch + (size_t)blockIdx.x * S::L;
+    double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + S::LDS_BYTES);
+    unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
+    float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
+    fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
+    inv_rs<N, RING32>(a, lds_all, tw, scr, hsum, hcnt);
+}
+
+// lengths with a role-split kernel (square, at least two tile pairs per pass)
+template <int N1, int N2>
+constexpr bool has_rs()
+{
+    return N1 == N2 && (N1 == 128 || N1 == 256);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -215,6 +211,15 @@ static hipError_t launch_inv_t(const InvArgs &a, int grid, hipStream_t st)
     const bool ring32 = (unsigned long long)a.FBS * S::L * sizeof(float2) < (1ull << 32);
     const int var = a.tangent_ref ? VAR_TANGENT : a.out_f16 ? VAR_F16 : VAR_PLAIN;
 #define LAUNCH(R, V) hipLaunchKernelGGL((inv_kernel<N1, N2, R, V>), dim3(grid), dim3(WG), lds, st, a)
+    if constexpr (has_rs<N1, N2>()) {
+        if ((a.order & MSOSM_SCHEDULE_SPLIT) && var == VAR_PLAIN) {
+            if (ring32)
+                hipLaunchKernelGGL((inv_rs_kernel<N1, true>), dim3(grid), dim3(WG), lds, st, a);
+            else
+                hipLaunchKernelGGL((inv_rs_kernel<N1, false>), dim3(grid), dim3(WG), lds, st, a);
+            return hipGetLastError();
+        }
+    }
     if (ring32) {
         if (var == VAR_TANGENT) LAUNCH(true, VAR_TANGENT);
         else if (var == VAR_F16) LAUNCH(true, VAR_F16);
@@ -248,6 +253,17 @@ void split_length(int log2L, int *n1, int *n2)
 }
 
 bool length_supported(int log2L) { return log2L >= 12 && log2L <= 22; }
+
+bool split_supported(int log2L)
+{
+    switch (log2L) {
+#define X(lg, a, b)                                                                                                    \
+    case lg: return has_rs<a, b>();
+        MSOSM_SHAPES(X)
+#undef X
+    }
+    return false;
+}
 
 int tiles_lds_bytes(int log2L)
 {

@@ -111,6 +111,21 @@ __device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
     return DIR < 0 ? cconj(w) : w;
 }
 
+// Position of element (k1, n2) of the four-step intermediate in a workgroup's scratch slot (pass A
+// columns k1 < N1, rows n2 < N2; NC columns per pass-A tile, NC rows per pass-B tile).
+//   layout 0: [k1 / NC][n2 / NC][n2 % NC][k1 % NC]
+//   layout 1: [n2 / NC][k1 / NC][n2 % NC][k1 % NC]   (square lengths, N1 = N2)
+// Pass B's tile t (rows n2 / NC = t) of layout 0 occupies exactly the positions of pass A's tile t
+// (columns k1 / NC = t) of layout 1, and the other way round: a pass A writing one layout can take
+// over, tile by tile, the positions that a pass B reading the other layout has consumed. Both keep
+// runs of NC consecutive k1 (the lanes of a store or load) contiguous.
+template <int N2, int NC>
+__device__ __forceinline__ uint32_t slot_pos(int k1, int n2, int layout)
+{
+    if (layout == 0) return (uint32_t)(((k1 / NC) * N2 + n2) * NC + (k1 % NC));
+    return (uint32_t)((n2 / NC) * (N2 * NC) + (k1 / NC) * (NC * NC) + (n2 % NC) * NC + (k1 % NC));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Pass A: column sub-FFTs. load(k) returns input element k (k = k1 + N1*k2).
 // Output Z[k1][n2] * W^{DIR n2 k1} goes to scr[(tile * N2 + n2) * NCA + col].
@@ -128,7 +143,7 @@ __device__ __forceinline__ void pin16(float2 (&v)[16])
 // the barriers).
 template <int N1, int N2, int DIR, int NSUB>
 __device__ __forceinline__ void pass_a_tile(float2 (&v)[16], float2 *__restrict__ scr, float2 *lds_all, const Tw &tw,
-                                            int t0)
+                                            int t0, int layout = 0)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
@@ -154,7 +169,7 @@ __device__ __forceinline__ void pass_a_tile(float2 (&v)[16], float2 *__restrict_
 #pragma unroll
             for (int t = 0; t < R; ++t) {
                 const int n2 = final_pos<NS>(jt, q, t);
-                st32(scr, (uint32_t)((tile * N2 + n2) * NC + col), cmul(v[q * R + t], w));
+                st32(scr, slot_pos<N2, NC>(k1, n2, layout), cmul(v[q * R + t], w));
                 if (t + 1 < R) w = cmul(w, c); // depth <= 15 products: < 1e-6 relative
             }
         }
@@ -203,10 +218,12 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
 // Pass A of the dedispersion unit with the per-bin delays of the next tile in flight (16 VGPRs): at
 // each tile step the ring and chirp loads are issued at once, from delays already in registers, so a
 // step waits for one memory round trip instead of two dependent ones.
-template <int N1, int N2, bool RING32, bool POWER, int NSUB>
+// XB extra barriers follow each step (the role-split kernel matches the pass-B role's count).
+template <int N1, int N2, bool RING32, bool POWER, int NSUB, int XB = 0>
 __device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring, int FBS, int gm,
                                                  const float2 *__restrict__ Hd, const uint16_t *__restrict__ dl,
-                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, float &power)
+                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, float &power,
+                                                 int layout = 0)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int L = S::L, NC = S::NCA, TPC = N2 / 16, NT = S::NTA;
@@ -253,7 +270,9 @@ __device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring
             v[t] = cmul(x[t], h[t]);
             if constexpr (POWER) power += v[t].x * v[t].x + v[t].y * v[t].y;
         }
-        pass_a_tile<N1, N2, +1, NSUB>(v, scr, lds_all, tw, t0);
+        pass_a_tile<N1, N2, +1, NSUB>(v, scr, lds_all, tw, t0, layout);
+#pragma unroll
+        for (int i = 0; i < XB; ++i) __syncthreads();
     }
 }
 
@@ -266,16 +285,17 @@ __device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring
 // prep(tile, active) runs at the start of each tile step, before the next tile's prefetch is issued
 // (global loads the epilogue needs: they retire before the prefetch, so waiting for them does not
 // wait for it); its result is handed to emit(lds, tile, active, prepared).
-template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, class Prep, class Emit>
+// SUB0: the first of this pass's 256-thread groups (the role-split kernel runs pass B on groups 2, 3).
+template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, int SUB0 = 0, class Prep, class Emit>
 __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Prep &prep, const Emit &emit,
-                                       float2 *lds_all, const Tw &tw)
+                                       float2 *lds_all, const Tw &tw, int layout = 0)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
     auto fetch = [&](int t0, float2 *dst) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        const int sub = tid >> 8, lt = tid & 255;
+        const int sub = (tid >> 8) - SUB0, lt = tid & 255;
         const int jt = lt % TPC, row = lt / TPC;
         const int tile = t0 + sub;
         const bool active = (S::NTB % NSUB == 0) || tile < S::NTB;
@@ -284,7 +304,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const int k1 = jt + t * TPC;
-                dst[t] = ld32(scr, (uint32_t)(((k1 / NCA) * N2 + n2) * NCA + (k1 % NCA)));
+                dst[t] = ld32(scr, slot_pos<N2, NCA>(k1, n2, layout));
             }
         } else {
 #pragma unroll
@@ -297,9 +317,9 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
     for (int t0 = 0; t0 < S::NTB; t0 += NSUB) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        const int sub = tid >> 8, lt = tid & 255;
+        const int sub = (tid >> 8) - SUB0, lt = tid & 255;
         const int jt = lt % TPC, row = lt / TPC;
-        float2 *lds = lds_all + sub * S::TILE_WORDS;
+        float2 *lds = lds_all + (sub + SUB0) * S::TILE_WORDS;
         const int tile = t0 + sub;
         const bool active = (S::NTB % NSUB == 0) || tile < S::NTB;
         const auto prepared = prep(tile, active);
@@ -399,61 +419,37 @@ __device__ __forceinline__ void fold_lds_init(double *hsum, unsigned int *hcnt, 
 // 2 Re(conj(ref) y) against a primal series instead of |y|^2.
 constexpr int VAR_PLAIN = 0, VAR_F16 = 1, VAR_TANGENT = 2;
 
-// One dedispersion unit (DM d, block b): gather * H -> IDFT_L -> keep [M, L) * 1/L -> outputs and
-// fold. hsum/hcnt are the workgroup's LDS fold histogram (zero on entry and on exit), pw_lds the
-// FP16-scale reduction word (zero on entry and on exit).
-// With PF pass B prefetches its next scratch tile (32 VGPRs); with PFA the gather runs with the
-// next tile's delays in flight (16 VGPRs).
-template <int N1, int N2, bool RING32, int VAR, bool PF = false, int NSUB = SUBT, bool PFA = PF>
-__device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
-                                         float2 *__restrict__ scr, double *hsum, unsigned int *hcnt, float *pw_lds)
+// Moves the LDS fold histogram of DM d into the global fold (threads t0, t0 + nthr, ... of the
+// workgroup) and clears it for the next unit.
+__device__ __forceinline__ void fold_flush(const InvArgs &a, int d, double *hsum, unsigned int *hcnt, int t0, int nthr)
+{
+    double *gs = a.fold_sum + (size_t)d * a.nbins;
+    unsigned long long *gc = a.fold_cnt + (size_t)d * a.nbins;
+    for (int i = t0; i < a.nbins; i += nthr) {
+        const unsigned int c = hcnt[i];
+        if (c) {
+            unsafeAtomicAdd(&gs[i], hsum[i]);
+            atomicAdd(&gc[i], (unsigned long long)c);
+            hsum[i] = 0.0;
+            hcnt[i] = 0u;
+        }
+    }
+}
+
+// Pass B of the dedispersion unit (DM d, block b): IDFT rows from the scratch slot (written in
+// `layout`), then the kept half [M, L) to the outputs and into the fold histogram.
+template <int N1, int N2, int VAR, bool PF, int NSUB, int SUB0 = 0>
+__device__ __forceinline__ void inv_unit_pass_b(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
+                                                const float2 *__restrict__ scr, double *hsum, unsigned int *hcnt,
+                                                float inv_scale, int layout = 0)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int L = S::L, M = L / 2;
     const bool fold = a.fold_sum != nullptr;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
-    // a.H (and dH/dDM) carry the 1/L of the inverse transform: a power of two, so the series is the
-    // same bits as scaling after the transform, one multiply per sample cheaper
-    // opaque thread index: keeps thread-dependent values from being hoisted out of the caller's
-    // unit loop (where they would stay live, and spill, across units)
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    const int gm = (int)((a.g0 + b) % a.FBS); // ring slot of this block
-    const int FBS = a.FBS;
-    const float2 *__restrict__ Hd = a.H + (size_t)d * L;
-    const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
-    const float2 *__restrict__ ring = a.ring;
-    float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval; Y includes 1/L)
-    auto load = [&](int k) -> float2 {
-        int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
-        if (src < 0) src += FBS;
-        const float2 x = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + (uint32_t)k) : ring[(size_t)src * L + k];
-        const float2 y = cmul(x, ld32(Hd, (uint32_t)k));
-        if constexpr (f16) power += y.x * y.x + y.y * y.y;
-        return y;
-    };
-    if constexpr (PFA)
-        pass_a_gather_dl<N1, N2, RING32, f16, NSUB>(ring, FBS, gm, Hd, dl, scr, lds_all, tw, power);
-    else
-        pass_a<N1, N2, +1, PF, NSUB>(load, scr, lds_all, tw);
-    float inv_scale = 1.f, out_scale = 1.f;
-    if constexpr (f16) {
-        // block rms of the series over the L outputs: sqrt(sum |Y|^2) with Y = X H / L (Parseval);
-        // scale = 2^floor(log2(rms))
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
-        if ((tid & 63) == 0) atomicAdd(pw_lds, power);
-        __syncthreads();
-        const float tot = *pw_lds;
-        const float rms = sqrtf(tot);
-        int e2 = 0;
-        (void)frexpf(rms, &e2); // rms = m * 2^e2, m in [0.5, 1)
-        out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
-        inv_scale = 1.f / out_scale; // exact: a power of two
-        if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
-    }
-    __syncthreads();
     float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * M : nullptr;
     uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * M : nullptr;
     uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
@@ -530,22 +526,174 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
             atomicAdd(&hcnt[cur], ccnt);
         }
     };
-    pass_b<N1, N2, +1, PF, NSUB>(scr, prep, emit, lds_all, tw);
+    pass_b<N1, N2, +1, PF, NSUB, SUB0>(scr, prep, emit, lds_all, tw, layout);
+}
+
+// One dedispersion unit (DM d, block b): gather * H -> IDFT_L -> keep [M, L) * 1/L -> outputs and
+// fold. hsum/hcnt are the workgroup's LDS fold histogram (zero on entry and on exit), pw_lds the
+// FP16-scale reduction word (zero on entry and on exit).
+// With PF pass B prefetches its next scratch tile (32 VGPRs); with PFA the gather runs with the
+// next tile's delays in flight (16 VGPRs).
+template <int N1, int N2, bool RING32, int VAR, bool PF = false, int NSUB = SUBT, bool PFA = PF>
+__device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 *lds_all, const Tw &tw,
+                                         float2 *__restrict__ scr, double *hsum, unsigned int *hcnt, float *pw_lds)
+{
+    using S = Shape<N1, N2, NSUB>;
+    constexpr int L = S::L;
+    const bool fold = a.fold_sum != nullptr;
+    constexpr bool f16 = (VAR & VAR_F16) != 0;
+    // a.H (and dH/dDM) carry the 1/L of the inverse transform: a power of two, so the series is the
+    // same bits as scaling after the transform, one multiply per sample cheaper
+    // opaque thread index: keeps thread-dependent values from being hoisted out of the caller's
+    // unit loop (where they would stay live, and spill, across units)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int gm = (int)((a.g0 + b) % a.FBS); // ring slot of this block
+    const int FBS = a.FBS;
+    const float2 *__restrict__ Hd = a.H + (size_t)d * L;
+    const uint16_t *__restrict__ dl = a.delay + (size_t)d * L;
+    const float2 *__restrict__ ring = a.ring;
+    float power = 0.f; // sum |Y|^2 of this thread's bins (FP16 output scale, Parseval; Y includes 1/L)
+    auto load = [&](int k) -> float2 {
+        int src = gm - (int)ld32(dl, (uint32_t)k); // delays are < FBS, so one wrap at most
+        if (src < 0) src += FBS;
+        const float2 x = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + (uint32_t)k) : ring[(size_t)src * L + k];
+        const float2 y = cmul(x, ld32(Hd, (uint32_t)k));
+        if constexpr (f16) power += y.x * y.x + y.y * y.y;
+        return y;
+    };
+    if constexpr (PFA)
+        pass_a_gather_dl<N1, N2, RING32, f16, NSUB>(ring, FBS, gm, Hd, dl, scr, lds_all, tw, power);
+    else
+        pass_a<N1, N2, +1, PF, NSUB>(load, scr, lds_all, tw);
+    float inv_scale = 1.f, out_scale = 1.f;
+    if constexpr (f16) {
+        // block rms of the series over the L outputs: sqrt(sum |Y|^2) with Y = X H / L (Parseval);
+        // scale = 2^floor(log2(rms))
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
+        if ((tid & 63) == 0) atomicAdd(pw_lds, power);
+        __syncthreads();
+        const float tot = *pw_lds;
+        const float rms = sqrtf(tot);
+        int e2 = 0;
+        (void)frexpf(rms, &e2); // rms = m * 2^e2, m in [0.5, 1)
+        out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
+        inv_scale = 1.f / out_scale; // exact: a power of two
+        if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
+    }
+    __syncthreads();
+    inv_unit_pass_b<N1, N2, VAR, PF, NSUB>(a, d, b, lds_all, tw, scr, hsum, hcnt, inv_scale);
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f; // next unit's reduction (ordered by the barriers)
     if (fold) {
-        double *gs = a.fold_sum + (size_t)d * a.nbins;
-        unsigned long long *gc = a.fold_cnt + (size_t)d * a.nbins;
-        for (int i = tid; i < a.nbins; i += 256 * NSUB) {
-            const unsigned int c = hcnt[i];
-            if (c) {
-                unsafeAtomicAdd(&gs[i], hsum[i]);
-                atomicAdd(&gc[i], (unsigned long long)c);
-                hsum[i] = 0.0;
-                hcnt[i] = 0u;
-            }
-        }
+        fold_flush(a, d, hsum, hcnt, tid, 256 * NSUB);
         __syncthreads();
+    }
+}
+
+
+// Unit order of the persistent dedispersion kernels (a unit is one (DM, block) pair). Workgroup w
+// visits the sequence positions first, first + step, ... below end; a position names a unit or, at
+// the edges of the tile orders, nothing.
+//   tiles: the G/8 workgroups of an XCD run dg DMs x (G/8)/dg consecutive blocks at a time, so each
+//          DM's chirp and delays and each block's ring slots are shared in that XCD's L2;
+//   XCD DM-/block-major: workgroups w, w+8, ... share an XCD and get consecutive virtual ids;
+//   linear: workgroup w runs units w, w+G, ...
+struct UnitOrder {
+    int dg, bg, j, tb;
+    bool tiles, block_major;
+    long long first, end, step;
+    __device__ UnitOrder(const InvArgs &a, int G, int w)
+    {
+        const int order = a.order & MSOSM_ORDER_MASK;
+        dg = order == MSOSM_ORDER_XCD_TILE4 ? 4 : order == MSOSM_ORDER_XCD_TILE8 ? 8 : 0;
+        tiles = dg && G % 8 == 0 && (G / 8) % dg == 0;
+        const bool xcd = order != MSOSM_ORDER_LINEAR && G % 8 == 0;
+        block_major = order == MSOSM_ORDER_XCD_BLOCK_MAJOR;
+        bg = tiles ? (G / 8) / dg : 1;
+        j = w / 8;
+        tb = (a.B + bg - 1) / bg;
+        const int td = tiles ? (a.D + dg - 1) / dg : 0;
+        first = tiles ? w % 8 : xcd ? (w % 8) * (G / 8) + w / 8 : w;
+        end = tiles ? (long long)td * tb : (long long)a.D * a.B;
+        step = tiles ? 8 : G;
+    }
+    // the unit at position s; false when the position names none
+    __device__ bool at(long long s, const InvArgs &a, int &d, int &b) const
+    {
+        if (tiles) {
+            d = (int)(s / tb) * dg + j / bg;
+            b = (int)(s % tb) * bg + j % bg;
+        } else if (block_major) {
+            d = (int)(s % a.D);
+            b = (int)(s / a.D);
+        } else {
+            d = (int)(s / a.B);
+            b = (int)(s % a.B);
+        }
+        return d < a.D && b < a.B;
+    }
+    // the next unit from position s on (s moves past it); false at the end of the sequence
+    __device__ bool next(long long &s, const InvArgs &a, int &d, int &b) const
+    {
+        for (; s < end; s += step)
+            if (at(s, a, d, b)) {
+                s += step;
+                return true;
+            }
+        return false;
+    }
+};
+
+// Role-split dedispersion of square lengths (N1 = N2 = N, plain epilogue). Waves 0-7 (256-thread
+// groups 0, 1) run pass A of the workgroup's unit j while waves 8-15 (groups 2, 3) run pass B of unit
+// j-1 on the same scratch slot: units alternate the slot layout, so pass A writes each tile pair
+// exactly where pass B has just read its own (slot_pos), and the memory-heavy gather and the
+// slot read-back of one role overlap the other role's transforms. Both roles pass the same
+// sequence of barriers: per phase (one unit of each role), STEPS tile steps of 4 barriers (pass A's
+// sub-FFT exchange and 2 more; pass B's exchange and its sample-order write) and 2 at the phase end,
+// between which the pass-B role moves its fold histogram out. A role without a unit in a phase (the
+// pass-B role in the first, the pass-A role in the last) only passes the barriers.
+template <int N, bool RING32>
+__device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const Tw &tw, float2 *__restrict__ scr,
+                                       double *hsum, unsigned int *hcnt)
+{
+    using S = Shape<N, N, 2>;
+    static_assert(S::NTA == S::NTB && S::NCA == S::NCB && S::NTA >= 4, "role split: square lengths, 4+ tiles");
+    constexpr int STEPS = S::NTA / 2, L = S::L;
+    const UnitOrder o(a, gridDim.x, blockIdx.x);
+    const int role = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 9);
+    long long s = o.first;
+    int d = 0, b = 0, layout = 0;
+    if (role == 0) {
+        for (;;) {
+            const bool ok = o.next(s, a, d, b);
+            if (ok) {
+                const int gm = (int)((a.g0 + b) % a.FBS);
+                float power = 0.f;
+                pass_a_gather_dl<N, N, RING32, false, 2, 2>(a.ring, a.FBS, gm, a.H + (size_t)d * L,
+                                                           a.delay + (size_t)d * L, scr, lds_all, tw, power, layout);
+            } else {
+                for (int i = 0; i < 4 * STEPS; ++i) __syncthreads();
+            }
+            __syncthreads(); // phase end (the pass-B role flushes its fold in between)
+            __syncthreads();
+            if (!ok) break;
+            layout ^= 1;
+        }
+    } else {
+        for (int i = 0; i < 4 * STEPS; ++i) __syncthreads(); // phase 0: pass A of the first unit
+        __syncthreads();
+        __syncthreads();
+        const bool fold = a.fold_sum != nullptr;
+        while (o.next(s, a, d, b)) {
+            inv_unit_pass_b<N, N, VAR_PLAIN, true, 2, 2>(a, d, b, lds_all, tw, scr, hsum, hcnt, 1.f, layout);
+            __syncthreads();
+            if (fold) fold_flush(a, d, hsum, hcnt, (int)threadIdx.x - 512, 512);
+            __syncthreads();
+            layout ^= 1;
+        }
     }
 }
 

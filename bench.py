@@ -23,6 +23,16 @@ for p in (ROOT, os.path.join(ROOT, "acceleratedms-osm_amd")):
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 ORDERS = ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8"]
+SPLIT = 8  # schedule flag: role-split launch (MSOSM_SCHEDULE_SPLIT)
+
+
+def order_name(order: int) -> str:
+    return ORDERS[order & 7] + ("+split" if order & SPLIT else "")
+
+
+def order_code(name: str) -> int:
+    base, _, split = name.partition("+")
+    return ORDERS.index(base) | (SPLIT if split == "split" else 0)
 
 # Uneven grid (--workload uneven): complex64 baseband, 2^26 samples at 32 MS/s, band 784-816 MHz,
 # 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
@@ -85,7 +95,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
     ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
-    ap.add_argument("--order", choices=ORDERS, default=None,
+    ap.add_argument("--order", choices=ORDERS + [o + "+split" for o in ORDERS], default=None,
                     help="fixed unit order instead of the autotuner (profiling runs)")
     ap.add_argument("--refine-steps", type=int, default=40)
     ap.add_argument("--push-blocks", type=int, default=0,
@@ -149,13 +159,13 @@ def main():
     log(f"plan: FBS={info.FBS} workgroups={info.workgroups} device bytes={info.device_bytes / 2**30:.2f} GiB")
     base = data.data_ptr()
     if args.order:
-        plan.set_schedule(0, ORDERS.index(args.order))
+        plan.set_schedule(0, order_code(args.order))
         wg, order = plan.schedule()
-        schedule = {"workgroups": wg, "order": ORDERS[order], "autotuned": False}
+        schedule = {"workgroups": wg, "order": order_name(order), "autotuned": False}
     else:
         # tile schedule: measured on the first pushes of the record (this resets the plan and the fold)
         wg, order, tms = plan.autotune(base, 3)
-        schedule = {"workgroups": wg, "order": ORDERS[order], "autotuned": True}
+        schedule = {"workgroups": wg, "order": order_name(order), "autotuned": True}
         log(f"autotune: {schedule} ({tms:.3f} ms per dedispersion launch)")
 
     def step():

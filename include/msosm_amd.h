@@ -160,14 +160,21 @@ enum {
     MSOSM_ORDER_LINEAR = 1,         /* workgroup w runs units w, w+G, ... in DM-major order */
     MSOSM_ORDER_XCD_BLOCK_MAJOR = 2, /* as the default, but consecutive units are one block of consecutive DMs */
     MSOSM_ORDER_XCD_TILE4 = 3,       /* an XCD's workgroups run 4 DMs x (workgroups/32) blocks at a time */
-    MSOSM_ORDER_XCD_TILE8 = 4        /* the same with 8 DMs */
+    MSOSM_ORDER_XCD_TILE8 = 4,       /* the same with 8 DMs */
+    MSOSM_ORDER_MASK = 7,
+    /* flag on the order: role-split launch, where half of each workgroup runs the first transform pass
+     * of one unit while the other half runs the second pass of the previous unit on the same scratch
+     * slot (FFT lengths 2^14 and 2^16, plain epilogue: complex64/uint16 output and/or fused fold;
+     * other plans run the sequential form) */
+    MSOSM_SCHEDULE_SPLIT = 8
 };
 int msosm_plan_set_schedule(msosm_plan *plan, int workgroups, int order);
 int msosm_plan_get_schedule(msosm_plan *plan, int *workgroups, int *order);
 /* Autotuner: times the dedispersion launch of every schedule candidate (unit orders x workgroup
- * counts {max, max/2}) over `pushes` pushes of device input (batch*M samples each, in the plan's
- * input format; the first push of each candidate is not timed), applies the fastest and returns it.
- * The tuning pushes consume the plan's state: the plan and its attached fold are reset afterwards. */
+ * counts {max, max/2}, each also with MSOSM_SCHEDULE_SPLIT where the plan supports it) over
+ * `pushes` pushes of device input (batch*M samples each, in the plan's input format; the first push
+ * of each candidate is not timed), applies the fastest and returns it. The tuning pushes consume the
+ * plan's state: the plan and its attached fold are reset afterwards. */
 int msosm_plan_autotune(msosm_plan *plan, const void *input_dev, int pushes, int *best_workgroups,
                         int *best_order, double *best_ms);
 /* HIP-event timing of the plan's kernels. kernel: 0 = forward FFT, 1 = dedispersion */

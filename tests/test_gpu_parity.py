@@ -204,7 +204,8 @@ def test_schedules_are_bitwise_identical_and_autotune_resets():
     dev = torch.from_numpy(raw.view(np.float32)).cuda()
     outs = []
     for wg, order in [(0, ms.ORDER_XCD_DM_MAJOR), (0, ms.ORDER_LINEAR), (0, ms.ORDER_XCD_BLOCK_MAJOR),
-                      (0, ms.ORDER_XCD_TILE4), (0, ms.ORDER_XCD_TILE8), (16, 0), (64, ms.ORDER_XCD_TILE8)]:
+                      (0, ms.ORDER_XCD_TILE4), (0, ms.ORDER_XCD_TILE8), (16, 0), (64, ms.ORDER_XCD_TILE8),
+                      (0, ms.ORDER_LINEAR | ms.SCHEDULE_SPLIT), (8, ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_SPLIT)]:
         plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
         plan.set_schedule(wg, order)
         o = []
@@ -225,3 +226,41 @@ def test_schedules_are_bitwise_identical_and_autotune_resets():
         o.append(plan.output_c64())
     plan.close()
     assert np.array_equal(outs[0], np.concatenate(o, axis=1))
+
+
+@pytest.mark.parametrize("band", ["L2^14", "L2^16"])
+def test_role_split_matches_sequential(band):
+    # the role-split launch (one half of each workgroup on pass A of a unit, the other on pass B of the
+    # previous unit, alternating slot layouts) against the sequential form: identical series bits and
+    # fold counts, with workgroups running up to 7 units each (every layout change exercised)
+    import torch
+
+    bw, f0, dms = (BW, F0, [5.0, 25.0, 50.0]) if band == "L2^14" else (64e6, 1218e6, [10.0, 50.0, 100.0])
+    batch, calls, nbins, period = 4, 3, 256, 0.0011
+    design = ms.Design(bw, f0, dms)
+    assert 2 * design.M == (1 << 14 if band == "L2^14" else 1 << 16)
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), bw, f0, float(max(dms)), period, seed=43)
+    dev = torch.from_numpy(np.ascontiguousarray(raw[:n]).view(np.float32)).cuda()
+    res = []
+    for wg, order in [(8, ms.ORDER_LINEAR), (8, ms.ORDER_LINEAR | ms.SCHEDULE_SPLIT),
+                      (16, ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_SPLIT), (0, ms.ORDER_XCD_TILE4 | ms.SCHEDULE_SPLIT)]:
+        print(f"case {band} workgroups={wg} order={order}", flush=True)
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
+        fold = ms.Fold(period, bw, per, nbins, num_series=len(dms))
+        plan.attach_fold(fold)
+        plan.set_schedule(wg, order)
+        o = []
+        for c in range(calls):
+            plan.push(dev.data_ptr() + c * per * 8)
+            o.append(plan.output_c64())
+        _, sums, cnts = fold.read()
+        res.append((np.concatenate(o, axis=1), sums, cnts))
+        plan.close()
+        fold.close()
+    ref = res[0]
+    for series, sums, cnts in res[1:]:
+        assert np.array_equal(series, ref[0])
+        assert np.array_equal(cnts, ref[2])
+        np.testing.assert_allclose(sums, ref[1], rtol=1e-12)  # FP64 sums: atomic order only
