@@ -119,6 +119,11 @@ __device__ __forceinline__ float2 big_twiddle(const Tw &tw, int m)
 // (columns k1 / NC = t) of layout 1, and the other way round: a pass A writing one layout can take
 // over, tile by tile, the positions that a pass B reading the other layout has consumed. Both keep
 // runs of NC consecutive k1 (the lanes of a store or load) contiguous.
+template <int V>
+struct IntC {
+    static constexpr int value = V;
+};
+
 template <int N2, int NC>
 __device__ __forceinline__ uint32_t slot_pos(int k1, int n2, int layout)
 {
@@ -271,6 +276,67 @@ __device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring
             if constexpr (POWER) power += v[t].x * v[t].x + v[t].y * v[t].y;
         }
         pass_a_tile<N1, N2, +1, NSUB>(v, scr, lds_all, tw, t0, layout);
+#pragma unroll
+        for (int i = 0; i < XB; ++i) __syncthreads();
+    }
+}
+
+// Pass A of the role-split kernel: the ring values of the next tile step are loaded right after this
+// step's slot stores, so they are in flight across the step's last barriers (where the pass-B role
+// writes and emits its tile) instead of being waited for at the start of the next step; the chirp
+// values (mostly L2 hits) are loaded at the step start with the delays of the step after. The 32
+// VGPRs of the early ring values are free at that point (the transform's values are stored). XB
+// extra barriers follow each step.
+template <int N1, int N2, bool RING32, int NSUB, int XB>
+__device__ __forceinline__ void pass_a_gather_pf(const float2 *__restrict__ ring, int FBS, int gm,
+                                                 const float2 *__restrict__ Hd, const uint16_t *__restrict__ dl,
+                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, int layout)
+{
+    using S = Shape<N1, N2, NSUB>;
+    constexpr int L = S::L, NC = S::NCA, TPC = N2 / 16, NT = S::NTA;
+    static_assert(NT % NSUB == 0, "whole tile steps");
+    int dd[16];
+    float2 x[16];
+    auto first_bin = [&](int t0) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lt = tid & 255, tile = t0 + (tid >> 8);
+        return (uint32_t)(lt % NC + N1 * (lt / NC) + tile * NC);
+    };
+    auto load_delays = [&](int t0) {
+        const uint32_t k0 = first_bin(t0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) dd[t] = (int)ld32(dl, k0 + (uint32_t)(N1 * TPC * t));
+    };
+    auto gather = [&](int t0) {
+        const uint32_t k0 = first_bin(t0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const uint32_t k = k0 + (uint32_t)(N1 * TPC * t);
+            int src = gm - dd[t]; // delays are < FBS, so one wrap at most
+            if (src < 0) src += FBS;
+            x[t] = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + k) : ring[(size_t)src * L + k];
+        }
+    };
+    load_delays(0);
+    gather(0);
+#pragma unroll 1
+    for (int t0 = 0; t0 < NT; t0 += NSUB) {
+        float2 h[16];
+        {
+            const uint32_t k0 = first_bin(t0);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) h[t] = ld32(Hd, k0 + (uint32_t)(N1 * TPC * t));
+        }
+        if (t0 + NSUB < NT) load_delays(t0 + NSUB);
+        float2 v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = cmul(x[t], h[t]);
+        pass_a_tile<N1, N2, +1, NSUB>(v, scr, lds_all, tw, t0, layout);
+        // compiler-only memory barrier: the next loads may not move above the stores (with the
+        // restrict pointers they could, and would then be live across the transform)
+        asm volatile("" ::: "memory");
+        if (t0 + NSUB < NT) gather(t0 + NSUB);
 #pragma unroll
         for (int i = 0; i < XB; ++i) __syncthreads();
     }
@@ -665,34 +731,40 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
     const UnitOrder o(a, gridDim.x, blockIdx.x);
     const int role = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 9);
     long long s = o.first;
-    int d = 0, b = 0, layout = 0;
+    int d = 0, b = 0;
+    // the unit loops are unrolled by two, one body per slot layout, so that every slot address is
+    // a compile-time function of the thread's indices (a runtime layout costs 8 % more VALU)
     if (role == 0) {
-        for (;;) {
+        auto unit = [&](auto layout) -> bool {
             const bool ok = o.next(s, a, d, b);
             if (ok) {
                 const int gm = (int)((a.g0 + b) % a.FBS);
-                float power = 0.f;
-                pass_a_gather_dl<N, N, RING32, false, 2, 2>(a.ring, a.FBS, gm, a.H + (size_t)d * L,
-                                                           a.delay + (size_t)d * L, scr, lds_all, tw, power, layout);
+                pass_a_gather_pf<N, N, RING32, 2, 2>(a.ring, a.FBS, gm, a.H + (size_t)d * L, a.delay + (size_t)d * L,
+                                                     scr, lds_all, tw, decltype(layout)::value);
             } else {
                 for (int i = 0; i < 4 * STEPS; ++i) __syncthreads();
             }
             __syncthreads(); // phase end (the pass-B role flushes its fold in between)
             __syncthreads();
-            if (!ok) break;
-            layout ^= 1;
+            return ok;
+        };
+        while (unit(IntC<0>{}) && unit(IntC<1>{})) {
         }
     } else {
         for (int i = 0; i < 4 * STEPS; ++i) __syncthreads(); // phase 0: pass A of the first unit
         __syncthreads();
         __syncthreads();
         const bool fold = a.fold_sum != nullptr;
-        while (o.next(s, a, d, b)) {
-            inv_unit_pass_b<N, N, VAR_PLAIN, true, 2, 2>(a, d, b, lds_all, tw, scr, hsum, hcnt, 1.f, layout);
+        auto unit = [&](auto layout) -> bool {
+            if (!o.next(s, a, d, b)) return false;
+            inv_unit_pass_b<N, N, VAR_PLAIN, true, 2, 2>(a, d, b, lds_all, tw, scr, hsum, hcnt, 1.f,
+                                                         decltype(layout)::value);
             __syncthreads();
             if (fold) fold_flush(a, d, hsum, hcnt, (int)threadIdx.x - 512, 512);
             __syncthreads();
-            layout ^= 1;
+            return true;
+        };
+        while (unit(IntC<0>{}) && unit(IntC<1>{})) {
         }
     }
 }
