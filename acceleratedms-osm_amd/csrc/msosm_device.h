@@ -285,41 +285,45 @@ __device__ __forceinline__ void pass_a_gather_dl(const float2 *__restrict__ ring
 // step's slot stores, so they are in flight across the step's last barriers (where the pass-B role
 // writes and emits its tile) instead of being waited for at the start of the next step; the chirp
 // values (mostly L2 hits) are loaded at the step start with the delays of the step after. The 32
-// VGPRs of the early ring values are free at that point (the transform's values are stored). XB
-// extra barriers follow each step.
+// VGPRs of the early ring values are free at that point (the transform's values are stored). The
+// last step loads the first delays and ring values of the workgroup's next unit (next: there is
+// one, at ring slot gm_next with delay table dl_next), so x and dd carry over from one call to the
+// next and only the first unit (first) loads its own. XB extra barriers follow each step.
 template <int N1, int N2, bool RING32, int NSUB, int XB>
 __device__ __forceinline__ void pass_a_gather_pf(const float2 *__restrict__ ring, int FBS, int gm,
                                                  const float2 *__restrict__ Hd, const uint16_t *__restrict__ dl,
-                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, int layout)
+                                                 float2 *__restrict__ scr, float2 *lds_all, const Tw &tw, int layout,
+                                                 float2 (&x)[16], int (&dd)[16], bool first, bool next, int gm_next,
+                                                 const uint16_t *__restrict__ dl_next)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int L = S::L, NC = S::NCA, TPC = N2 / 16, NT = S::NTA;
     static_assert(NT % NSUB == 0, "whole tile steps");
-    int dd[16];
-    float2 x[16];
     auto first_bin = [&](int t0) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int lt = tid & 255, tile = t0 + (tid >> 8);
         return (uint32_t)(lt % NC + N1 * (lt / NC) + tile * NC);
     };
-    auto load_delays = [&](int t0) {
+    auto load_delays = [&](const uint16_t *__restrict__ tab, int t0) {
         const uint32_t k0 = first_bin(t0);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) dd[t] = (int)ld32(dl, k0 + (uint32_t)(N1 * TPC * t));
+        for (int t = 0; t < 16; ++t) dd[t] = (int)ld32(tab, k0 + (uint32_t)(N1 * TPC * t));
     };
-    auto gather = [&](int t0) {
+    auto gather = [&](int g, int t0) {
         const uint32_t k0 = first_bin(t0);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const uint32_t k = k0 + (uint32_t)(N1 * TPC * t);
-            int src = gm - dd[t]; // delays are < FBS, so one wrap at most
+            int src = g - dd[t]; // delays are < FBS, so one wrap at most
             if (src < 0) src += FBS;
             x[t] = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + k) : ring[(size_t)src * L + k];
         }
     };
-    load_delays(0);
-    gather(0);
+    if (first) {
+        load_delays(dl, 0);
+        gather(gm, 0);
+    }
 #pragma unroll 1
     for (int t0 = 0; t0 < NT; t0 += NSUB) {
         float2 h[16];
@@ -328,7 +332,11 @@ __device__ __forceinline__ void pass_a_gather_pf(const float2 *__restrict__ ring
 #pragma unroll
             for (int t = 0; t < 16; ++t) h[t] = ld32(Hd, k0 + (uint32_t)(N1 * TPC * t));
         }
-        if (t0 + NSUB < NT) load_delays(t0 + NSUB);
+        const bool last = t0 + NSUB >= NT;
+        if (!last)
+            load_delays(dl, t0 + NSUB);
+        else if (next)
+            load_delays(dl_next, 0);
         float2 v[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) v[t] = cmul(x[t], h[t]);
@@ -336,7 +344,10 @@ __device__ __forceinline__ void pass_a_gather_pf(const float2 *__restrict__ ring
         // compiler-only memory barrier: the next loads may not move above the stores (with the
         // restrict pointers they could, and would then be live across the transform)
         asm volatile("" ::: "memory");
-        if (t0 + NSUB < NT) gather(t0 + NSUB);
+        if (!last)
+            gather(gm, t0 + NSUB);
+        else if (next)
+            gather(gm_next, 0);
 #pragma unroll
         for (int i = 0; i < XB; ++i) __syncthreads();
     }
@@ -733,20 +744,32 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
     long long s = o.first;
     int d = 0, b = 0;
     // the unit loops are unrolled by two, one body per slot layout, so that every slot address is
-    // a compile-time function of the thread's indices (a runtime layout costs 8 % more VALU)
+    // a compile-time function of the thread's indices (a runtime layout costs 8 % more VALU). The
+    // pass-A role looks one unit ahead and starts its loads during the current unit's last tile step
+    // (the ring and delay tables are read-only); the pass-B role cannot, since the first tile of its
+    // next unit depends on every column that pass A stores in the same phase.
+    bool have = o.next(s, a, d, b), first = true;
     if (role == 0) {
+        int dd[16];
+        float2 x[16];
         auto unit = [&](auto layout) -> bool {
-            const bool ok = o.next(s, a, d, b);
-            if (ok) {
-                const int gm = (int)((a.g0 + b) % a.FBS);
-                pass_a_gather_pf<N, N, RING32, 2, 2>(a.ring, a.FBS, gm, a.H + (size_t)d * L, a.delay + (size_t)d * L,
-                                                     scr, lds_all, tw, decltype(layout)::value);
-            } else {
-                for (int i = 0; i < 4 * STEPS; ++i) __syncthreads();
+            if (!have) {
+                for (int i = 0; i < 4 * STEPS + 2; ++i) __syncthreads(); // the pass-B role's last unit
+                return false;
             }
+            int dn = 0, bn = 0;
+            const bool next = o.next(s, a, dn, bn);
+            const int gm = (int)((a.g0 + b) % a.FBS), gmn = (int)((a.g0 + bn) % a.FBS);
+            pass_a_gather_pf<N, N, RING32, 2, 2>(a.ring, a.FBS, gm, a.H + (size_t)d * L, a.delay + (size_t)d * L,
+                                                 scr, lds_all, tw, decltype(layout)::value, x, dd, first, next, gmn,
+                                                 a.delay + (size_t)dn * L);
             __syncthreads(); // phase end (the pass-B role flushes its fold in between)
             __syncthreads();
-            return ok;
+            first = false;
+            d = dn;
+            b = bn;
+            have = next;
+            return true;
         };
         while (unit(IntC<0>{}) && unit(IntC<1>{})) {
         }
@@ -756,12 +779,13 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
         __syncthreads();
         const bool fold = a.fold_sum != nullptr;
         auto unit = [&](auto layout) -> bool {
-            if (!o.next(s, a, d, b)) return false;
+            if (!have) return false;
             inv_unit_pass_b<N, N, VAR_PLAIN, true, 2, 2>(a, d, b, lds_all, tw, scr, hsum, hcnt, 1.f,
                                                          decltype(layout)::value);
             __syncthreads();
             if (fold) fold_flush(a, d, hsum, hcnt, (int)threadIdx.x - 512, 512);
             __syncthreads();
+            have = o.next(s, a, d, b);
             return true;
         };
         while (unit(IntC<0>{}) && unit(IntC<1>{})) {

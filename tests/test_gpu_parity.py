@@ -232,7 +232,8 @@ def test_schedules_are_bitwise_identical_and_autotune_resets():
 def test_role_split_matches_sequential(band):
     # the role-split launch (one half of each workgroup on pass A of a unit, the other on pass B of the
     # previous unit, alternating slot layouts) against the sequential form: identical series bits and
-    # fold counts, with workgroups running up to 7 units each (every layout change exercised)
+    # fold counts, with workgroups running up to 12 units each (every layout change, and every unit
+    # of a push chained through one workgroup's look-ahead)
     import torch
 
     bw, f0, dms = (BW, F0, [5.0, 25.0, 50.0]) if band == "L2^14" else (64e6, 1218e6, [10.0, 50.0, 100.0])
@@ -245,7 +246,8 @@ def test_role_split_matches_sequential(band):
     dev = torch.from_numpy(np.ascontiguousarray(raw[:n]).view(np.float32)).cuda()
     res = []
     for wg, order in [(8, ms.ORDER_LINEAR), (8, ms.ORDER_LINEAR | ms.SCHEDULE_SPLIT),
-                      (16, ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_SPLIT), (0, ms.ORDER_XCD_TILE4 | ms.SCHEDULE_SPLIT)]:
+                      (1, ms.ORDER_LINEAR | ms.SCHEDULE_SPLIT), (16, ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_SPLIT),
+                      (0, ms.ORDER_XCD_TILE4 | ms.SCHEDULE_SPLIT)]:
         print(f"case {band} workgroups={wg} order={order}", flush=True)
         plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
         fold = ms.Fold(period, bw, per, nbins, num_series=len(dms))
