@@ -195,10 +195,14 @@ def main():
     inv_avg_s = inv_ms / inv_n / 1e3
     alg_bytes = D * B * M * 16.0
     achieved = alg_bytes / inv_avg_s
+    # the role-split schedule launches inv_rs_kernel, the others inv_kernel; each has its own
+    # counter-derived traffic file under profiles/
+    kname = "inv_rs_kernel" if order & SPLIT else "inv_kernel"
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "inv_kernel<256,256>",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "kernel": "inv_rs_kernel<256>" if order & SPLIT else "inv_kernel<256,256>",
             "avg_launch_ms": inv_avg_s * 1e3, "launches": inv_n, "alg_bytes_per_launch": alg_bytes}
-    tpath = os.path.join(ROOT, "profiles", "traffic_msosm_dm_grid_int8.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_msosm_dm_grid_int8.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
