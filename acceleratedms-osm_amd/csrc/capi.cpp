@@ -847,7 +847,7 @@ extern "C" int msosm_plan_set_schedule(msosm_plan *p, int workgroups, int order)
     if (!p) return set_error(MSOSM_ERR_ARG, "plan_set_schedule: null plan");
     if (workgroups < 0 || workgroups > p->G)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: workgroups must be in [0, " + std::to_string(p->G) + "]");
-    if ((order & ~(MSOSM_ORDER_MASK | MSOSM_SCHEDULE_SPLIT)) || (order & MSOSM_ORDER_MASK) > MSOSM_ORDER_XCD_TILE8)
+    if ((order & ~(MSOSM_ORDER_MASK | MSOSM_SCHEDULE_SPLIT)) || (order & MSOSM_ORDER_MASK) > MSOSM_ORDER_XCD_TILE2)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: unknown order");
     p->sched_wg = workgroups;
     p->order = order;
@@ -880,8 +880,8 @@ extern "C" int msosm_plan_autotune(msosm_plan *p, const void *input, int pushes,
     // role-split candidates where the launch has that form (FFT length, plain epilogue)
     const int nsplit = split_supported(p->log2L) && !(p->out_flags & MSOSM_OUT_F16) ? 2 : 1;
     for (int wi = 0; wi < (half < p->G ? 2 : 1); ++wi)
-        for (int c = 0; c < (MSOSM_ORDER_XCD_TILE8 + 1) * nsplit; ++c) {
-            const int o = (c % (MSOSM_ORDER_XCD_TILE8 + 1)) | (c > MSOSM_ORDER_XCD_TILE8 ? MSOSM_SCHEDULE_SPLIT : 0);
+        for (int c = 0; c < (MSOSM_ORDER_XCD_TILE2 + 1) * nsplit; ++c) {
+            const int o = (c % (MSOSM_ORDER_XCD_TILE2 + 1)) | (c > MSOSM_ORDER_XCD_TILE2 ? MSOSM_SCHEDULE_SPLIT : 0);
             p->sched_wg = wgs[wi];
             p->order = o;
             int rc = msosm_plan_reset(p);

@@ -684,7 +684,11 @@ struct UnitOrder {
     __device__ UnitOrder(const InvArgs &a, int G, int w)
     {
         const int order = a.order & MSOSM_ORDER_MASK;
-        dg = order == MSOSM_ORDER_XCD_TILE4 ? 4 : order == MSOSM_ORDER_XCD_TILE8 ? 8 : 0;
+        dg = order == MSOSM_ORDER_XCD_TILE2    ? 2
+             : order == MSOSM_ORDER_XCD_TILE4  ? 4
+             : order == MSOSM_ORDER_XCD_TILE8  ? 8
+             : order == MSOSM_ORDER_XCD_TILE16 ? 16
+                                               : 0;
         tiles = dg && G % 8 == 0 && (G / 8) % dg == 0;
         const bool xcd = order != MSOSM_ORDER_LINEAR && G % 8 == 0;
         block_major = order == MSOSM_ORDER_XCD_BLOCK_MAJOR;

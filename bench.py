@@ -22,7 +22,7 @@ for p in (ROOT, os.path.join(ROOT, "acceleratedms-osm_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
-ORDERS = ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8"]
+ORDERS = ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8", "xcd_tile16", "xcd_tile2"]
 SPLIT = 8  # schedule flag: role-split launch (MSOSM_SCHEDULE_SPLIT)
 
 

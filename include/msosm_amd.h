@@ -161,6 +161,8 @@ enum {
     MSOSM_ORDER_XCD_BLOCK_MAJOR = 2, /* as the default, but consecutive units are one block of consecutive DMs */
     MSOSM_ORDER_XCD_TILE4 = 3,       /* an XCD's workgroups run 4 DMs x (workgroups/32) blocks at a time */
     MSOSM_ORDER_XCD_TILE8 = 4,       /* the same with 8 DMs */
+    MSOSM_ORDER_XCD_TILE16 = 5,      /* the same with 16 DMs */
+    MSOSM_ORDER_XCD_TILE2 = 6,       /* the same with 2 DMs */
     MSOSM_ORDER_MASK = 7,
     /* flag on the order: role-split launch, where half of each workgroup runs the first transform pass
      * of one unit while the other half runs the second pass of the previous unit on the same scratch

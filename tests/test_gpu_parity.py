@@ -204,7 +204,8 @@ def test_schedules_are_bitwise_identical_and_autotune_resets():
     dev = torch.from_numpy(raw.view(np.float32)).cuda()
     outs = []
     for wg, order in [(0, ms.ORDER_XCD_DM_MAJOR), (0, ms.ORDER_LINEAR), (0, ms.ORDER_XCD_BLOCK_MAJOR),
-                      (0, ms.ORDER_XCD_TILE4), (0, ms.ORDER_XCD_TILE8), (16, 0), (64, ms.ORDER_XCD_TILE8),
+                      (0, ms.ORDER_XCD_TILE4), (0, ms.ORDER_XCD_TILE8), (0, ms.ORDER_XCD_TILE16),
+                      (0, ms.ORDER_XCD_TILE2), (16, 0), (64, ms.ORDER_XCD_TILE8), (128, ms.ORDER_XCD_TILE16),
                       (0, ms.ORDER_LINEAR | ms.SCHEDULE_SPLIT), (8, ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_SPLIT)]:
         plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
         plan.set_schedule(wg, order)
