@@ -93,7 +93,9 @@ __global__ void __launch_bounds__(WG) inv_rs_kernel(InvArgs a)
     inv_rs<N, RING32>(a, lds_all, tw, scr, hsum, hcnt);
 }
 
-// lengths with a role-split kernel (square, at least two tile pairs per pass)
+// lengths with a role-split kernel: square, at least two tile pairs per pass, and rows of at most
+// 256 points (the pass-B prefetching form; at 512 x 512 the split gave wrong series and was slower,
+// 8.7-9.2 against 7.7-7.9 ms per launch, and is not built)
 template <int N1, int N2>
 constexpr bool has_rs()
 {
@@ -390,6 +392,21 @@ NodeDesc fill_node(long long nbytes)
 
 int workgroup_threads() { return WG; }
 
+// the role-split instances of a length take the dedispersion kernel's LDS size
+template <int N1, int N2>
+static hipError_t configure_rs_lds(int inv_bytes)
+{
+    if constexpr (has_rs<N1, N2>()) {
+        for (const void *f : {reinterpret_cast<const void *>(&inv_rs_kernel<N1, true>),
+                              reinterpret_cast<const void *>(&inv_rs_kernel<N1, false>)}) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, inv_bytes);
+            if (e != hipSuccess) return e;
+        }
+    }
+    (void)inv_bytes;
+    return hipSuccess;
+}
+
 hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes)
 {
     switch (log2L) {
@@ -409,7 +426,7 @@ hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes)
             e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, inv_bytes);                         \
             if (e != hipSuccess) return e;                                                                             \
         }                                                                                                              \
-        return hipSuccess;                                                                                             \
+        return configure_rs_lds<a_, b_>(inv_bytes);                                                                    \
     }
         MSOSM_SHAPES(X)
 #undef X

@@ -369,6 +369,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
+    static_assert(!PF || NS <= 256, "the prefetching form is validated for rows of up to 256 points");
     auto fetch = [&](int t0, float2 *dst) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
