@@ -178,6 +178,17 @@ struct PassRadix {
     static constexpr int R = (NS / Ns) >= 16 ? 16 : (NS / Ns);
 };
 
+// Number of radix passes of an NS-point sub-FFT. Every pass after the first starts with an LDS
+// exchange of two workgroup barriers.
+template <int NS, int Ns = 1>
+struct PassCount {
+    static constexpr int value = 1 + PassCount<NS, Ns * PassRadix<NS, Ns>::R>::value;
+};
+template <int NS>
+struct PassCount<NS, NS> {
+    static constexpr int value = 0;
+};
+
 // One pass: twiddle by e^{DIR 2 pi i t k/(Ns R)} (k = j mod Ns) and DFT_R, on the 16/R butterflies
 // a thread owns (j = jt + q * NS/16). Twiddle table tw[m] = e^{+2 pi i m / NS}.
 template <int NS, int R, int Ns, int DIR>

@@ -94,8 +94,8 @@ __global__ void __launch_bounds__(WG) inv_rs_kernel(InvArgs a)
 }
 
 // lengths with a role-split kernel: square, at least two tile pairs per pass, and rows of at most
-// 256 points, the lengths it is tested at (a 512 x 512 build gave wrong series, cause not found, and
-// was slower than the sequential form: 8.7-9.2 against 7.7-7.9 ms per launch)
+// 256 points (at 512 x 512 the split is correct but slower than the sequential form: 8.1-8.4
+// against 7.7-7.9 ms per launch)
 template <int N1, int N2>
 constexpr bool has_rs()
 {

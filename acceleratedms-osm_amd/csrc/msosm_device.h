@@ -369,7 +369,6 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
-    static_assert(!PF || NS <= 256, "the prefetching form is validated for rows of up to 256 points");
     auto fetch = [&](int t0, float2 *dst) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
@@ -733,8 +732,8 @@ struct UnitOrder {
 // j-1 on the same scratch slot: units alternate the slot layout, so pass A writes each tile pair
 // exactly where pass B has just read its own (slot_pos), and the memory-heavy gather and the
 // slot read-back of one role overlap the other role's transforms. Both roles pass the same
-// sequence of barriers: per phase (one unit of each role), STEPS tile steps of 4 barriers (pass A's
-// sub-FFT exchange and 2 more; pass B's exchange and its sample-order write) and 2 at the phase end,
+// sequence of barriers: per phase (one unit of each role), STEPS tile steps of BPS barriers (pass A's
+// sub-FFT exchanges and 2 more; pass B's exchanges and its sample-order write) and 2 at the phase end,
 // between which the pass-B role moves its fold histogram out. A role without a unit in a phase (the
 // pass-B role in the first, the pass-A role in the last) only passes the barriers.
 template <int N, bool RING32>
@@ -744,6 +743,8 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
     using S = Shape<N, N, 2>;
     static_assert(S::NTA == S::NTB && S::NCA == S::NCB && S::NTA >= 4, "role split: square lengths, 4+ tiles");
     constexpr int STEPS = S::NTA / 2, L = S::L;
+    // barriers of one tile step in either role: the sub-FFT's exchanges (two each) and 2 more
+    constexpr int BPS = 2 * (PassCount<N>::value - 1) + 2;
     const UnitOrder o(a, gridDim.x, blockIdx.x);
     const int role = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 9);
     long long s = o.first;
@@ -759,7 +760,7 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
         float2 x[16];
         auto unit = [&](auto layout) -> bool {
             if (!have) {
-                for (int i = 0; i < 4 * STEPS + 2; ++i) __syncthreads(); // the pass-B role's last unit
+                for (int i = 0; i < BPS * STEPS + 2; ++i) __syncthreads(); // the pass-B role's last unit
                 return false;
             }
             int dn = 0, bn = 0;
@@ -779,7 +780,7 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
         while (unit(IntC<0>{}) && unit(IntC<1>{})) {
         }
     } else {
-        for (int i = 0; i < 4 * STEPS; ++i) __syncthreads(); // phase 0: pass A of the first unit
+        for (int i = 0; i < BPS * STEPS; ++i) __syncthreads(); // phase 0: pass A of the first unit
         __syncthreads();
         __syncthreads();
         const bool fold = a.fold_sum != nullptr;
