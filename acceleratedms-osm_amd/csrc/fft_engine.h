@@ -188,6 +188,9 @@ template <int NS>
 struct PassCount<NS, NS> {
     static constexpr int value = 0;
 };
+static_assert(PassCount<16>::value == 1 && PassCount<128>::value == 2 && PassCount<256>::value == 2 &&
+                  PassCount<512>::value == 3 && PassCount<2048>::value == 3,
+              "radix-16 passes, the last one of radix NS / 16^k");
 
 // One pass: twiddle by e^{DIR 2 pi i t k/(Ns R)} (k = j mod Ns) and DFT_R, on the 16/R butterflies
 // a thread owns (j = jt + q * NS/16). Twiddle table tw[m] = e^{+2 pi i m / NS}.
