@@ -154,7 +154,10 @@ enum {
 int msosm_plan_set_launch_mode(msosm_plan *plan, int mode);
 
 /* Tile schedule of the persistent dedispersion launch (no reference counterpart): the number of
- * workgroups (0 = one per CU, rounded down to a multiple of 8; at most that) and the unit order. */
+ * workgroups (0 = one per CU, rounded down to a multiple of 8; at most that) and the unit order.
+ * A tile order whose tile does not divide the workgroups of an XCD (workgroups / 8) runs as the
+ * XCD-grouped DM-major order, and every XCD-grouped order runs as the linear one when the workgroups
+ * are not a multiple of 8. No schedule changes the output bits. */
 enum {
     MSOSM_ORDER_XCD_DM_MAJOR = 0,   /* the workgroups of one XCD run consecutive blocks of one DM (default) */
     MSOSM_ORDER_LINEAR = 1,         /* workgroup w runs units w, w+G, ... in DM-major order */
