@@ -57,6 +57,30 @@ struct MultiInvArgs {
     long long scr_stride;      // >= the largest L
     unsigned int *counter;     // unit counter, zero at launch
 };
+// Batched transforms of the GPU pulsar simulator (kernels_sim.hip): transform t reads element e at
+// src[t * in_t + e * in_e]; its output o goes to index di = t * out_t + o * out_e through the stage
+// `mode`.
+enum { SIM_PLAIN = 0, SIM_TWIDDLE = 1, SIM_CHIRP = 2, SIM_EXTRACT = 3 };
+struct SimFftArgs {
+    const float2 *src;
+    float2 *dst;
+    long long in_t, in_e, out_t, out_e;
+    long long ntrans;
+    int mode;
+    long long big_n;        // SIM_TWIDDLE: dst[di] = y * e^{DIR 2 pi i (o t mod big_n) / big_n}
+    const float2 *chirp;    // SIM_CHIRP: dst[di] = y * chirp[di]
+    long long ex_start;     // SIM_EXTRACT: for ex_start <= di < ex_start + ex_count, v = y * ex_scale
+    long long ex_count;     //   goes to ex_c64[di - ex_start] and, as the reference's uint16 pair,
+    float ex_scale;         //   to ex_u16[di - ex_start] (either may be null)
+    float2 *ex_c64;
+    uint32_t *ex_u16;
+    float2 *scratch;        // [grid][L]
+    const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
+    int tw_s;
+};
+bool sim_length_supported(int log2L);
+hipError_t launch_sim_fft(int log2L, int dir, const SimFftArgs &a, int grid, hipStream_t st);
+
 // A kernel launch as a graph node: function, launch shape and dynamic LDS (the arguments are the
 // launcher's, in order).
 struct NodeDesc {

@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+from typing import NamedTuple
 
 import numpy as np
 
@@ -43,6 +44,7 @@ EXPORTS = [
     "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
     "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune", "msosm_stream_wait",
+    "msosm_simulate_params", "msosm_simulate_noise", "msosm_simulate_chirp", "msosm_simulate_pulsar_block",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
@@ -155,6 +157,13 @@ def lib():
         "msosm_plan_get_schedule": ([vp, ip, ip], C.c_int),
         "msosm_plan_autotune": ([vp, vp, C.c_int, ip, ip, dp], C.c_int),
         "msosm_stream_wait": ([vp, vp], C.c_int),
+        "msosm_simulate_params": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong,
+                                   C.POINTER(C.c_ulonglong)], C.c_int),
+        "msosm_simulate_noise": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong, fp,
+                                  C.c_ulonglong], C.c_int),
+        "msosm_simulate_chirp": ([C.c_float, C.c_float, C.c_float, C.c_ulonglong, fp], C.c_int),
+        "msosm_simulate_pulsar_block": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong, C.c_int,
+                                         C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -605,3 +614,54 @@ class Saver:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU pulsar simulator (simulated_complex_signal.cpp:74-277)
+# ------------------------------------------------------------------------------------------------
+class SimParams(NamedTuple):
+    Np: int          # samples per period
+    Nd: int          # dispersion samples
+    range: int       # ceil(Nd / Np)
+    fftpoint: int    # dispersion FFT length per chunk of 128 periods
+    samples: int     # samples returned: (dummy_repeat + valid_repeat) * Np
+    noise_len: int   # complex samples of the noise buffer
+    start: int       # first extracted sample of a chunk's window
+    chunks: int      # chunks of 128 periods
+
+
+def simulate_params(bw: float, dm: float, f0: float, period: float, dummy_repeat: int = 0,
+                    valid_repeat: int = 1) -> SimParams:
+    v = (C.c_ulonglong * 8)()
+    _check(lib().msosm_simulate_params(bw, dm, f0, period, dummy_repeat, valid_repeat, v))
+    return SimParams(*[int(x) for x in v])
+
+
+def simulate_noise(bw: float, dm: float, f0: float, period: float, dummy_repeat: int = 0,
+                   valid_repeat: int = 1) -> np.ndarray:
+    """The undispersed noise periods the simulator transforms (complex64, noise_len samples)."""
+    p = simulate_params(bw, dm, f0, period, dummy_repeat, valid_repeat)
+    out = np.empty(p.noise_len, dtype=np.complex64)
+    _check(lib().msosm_simulate_noise(bw, dm, f0, period, dummy_repeat, valid_repeat, _fp(out.view(np.float32)),
+                                      p.noise_len))
+    return out
+
+
+def simulate_chirp(bw: float, dm: float, f0: float, fftpoint: int) -> np.ndarray:
+    out = np.empty(fftpoint, dtype=np.complex64)
+    _check(lib().msosm_simulate_chirp(bw, dm, f0, fftpoint, _fp(out.view(np.float32))))
+    return out
+
+
+def simulate_pulsar_block(bw: float, dm: float, f0: float, period: float, dummy_repeat: int = 0,
+                          valid_repeat: int = 1, device: int = 0, series: bool = False):
+    """The dispersed record as uint16 pairs (uint32, real in the low half); with series=True also the
+    complex64 series before the conversion."""
+    p = simulate_params(bw, dm, f0, period, dummy_repeat, valid_repeat)
+    codes = np.empty(p.samples, dtype=np.uint32)
+    c64 = np.empty(p.samples, dtype=np.complex64) if series else None
+    _check(lib().msosm_simulate_pulsar_block(bw, dm, f0, period, dummy_repeat, valid_repeat, device,
+                                             codes.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                             _fp(c64.view(np.float32)) if series else None, p.samples))
+    return (codes, c64) if series else codes
+

@@ -304,6 +304,31 @@ int msosm_saver_destroy(msosm_saver *saver);
 int msosm_saver_file_name(const char *dir, const char *prefix, const float *dms, int num_dms, int i, char *buf,
                           size_t cap);
 
+/* ---- GPU pulsar simulator: SimulatedComplexSignal::generate_pulsar_signal_block
+ * (simulated_pulsar/simulated_complex_signal.cpp:74-277) with the dispersion injected on the GPU.
+ * bw (= fs), dm, f0, period as the reference's float constructor arguments (:16-29);
+ * dummy_repeat + valid_repeat periods of Np samples are returned. The noise periods (seeds 1 and 2)
+ * and the chirp are built on the host as there; each chunk of 128 periods is dispersed with one
+ * complex FFT of fftpoint points on the device. Deviations: nothing is cached in or read from files
+ * (simulated_*.bin, random_reference.bin), and the noise buffer is zero-padded so that every chunk's
+ * window lies inside it (the reference reads past its buffer unless repeat is a multiple of 128). */
+#define MSOSM_SIM_NPARAMS 8
+/* out[8] = {Np, Nd, range, fftpoint, samples returned, noise buffer length, first extracted sample
+ * of a chunk's window, chunks} (:16-29, 80-116) */
+int msosm_simulate_params(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
+                          unsigned long valid_repeat, unsigned long long *out);
+/* the noise periods (generate_test_pulse, :43-72, 119-161) as interleaved complex64, len >= the
+ * noise buffer length (complex samples) */
+int msosm_simulate_noise(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
+                         unsigned long valid_repeat, float *periods, unsigned long long len);
+/* the dispersion chirp e^{+i phi} at fftpoint bins (:166-198), interleaved complex64 */
+int msosm_simulate_chirp(float bw, float dm, float f0, unsigned long long fftpoint, float *H);
+/* the simulated record (:200-262): uint16 pairs (x + 32768 truncated as on x86, real in the low
+ * half) and/or the complex64 series before the conversion, len >= repeat * Np; on `device` */
+int msosm_simulate_pulsar_block(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
+                                unsigned long valid_repeat, int device, uint32_t *out_u16, float *out_c64,
+                                unsigned long long len);
+
 #ifdef __cplusplus
 }
 #endif
