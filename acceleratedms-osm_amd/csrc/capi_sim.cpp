@@ -144,6 +144,7 @@ extern "C" int msosm_simulate_pulsar_block(float bw, float dm, float f0, float p
     sim_noise(p, periods.data());
     sim_chirp(bw, dm, f0, p.fftpoint, H.data(), host_threads());
 
+    if (device < 0) SIMCHK(hipGetDevice(&device)); // the caller's current device
     SIMCHK(hipSetDevice(device));
     SimDev d;
     SIMCHK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));

@@ -1,0 +1,51 @@
+// Reference-compatible SimulatedComplexSignal (simulated_pulsar/simulated_complex_signal.cpp:16-41,
+// 74-277): the constructor sets the reference's members; generate_pulsar_signal_block returns the same
+// record (noise and chirp as the reference builds them, dispersion on the current GPU).
+#include "simulated_complex_signal.h"
+
+#include <cstdlib>
+#include <cstring>
+
+SimulatedComplexSignal::SimulatedComplexSignal(float bw, float dm, float f0, float period, string mode)
+    : data_type(mode), bw(bw), fs(bw), kdm((float)(4.15 * 1e15)), dm(dm), f0(f0), w0((float)(2 * pi * f0)),
+      period(period)
+{
+    unsigned long long p[MSOSM_SIM_NPARAMS];
+    msosm::check(msosm_simulate_params(bw, dm, f0, period, 0, 1, p), "SimulatedComplexSignal");
+    Np = (unsigned long)p[0];
+    Nd = (unsigned long)p[1];
+    if (verbose) {
+        cout << "Test signal parameters:" << endl;
+        cout << "Bandwidth: " << bw << "Hz" << endl;
+        cout << "Dispersion measure: " << dm << "pc cm^-3" << endl;
+        cout << "Start frequency: " << f0 << "Hz" << endl;
+        cout << "Pulsar period: " << period << "s" << endl;
+        cout << "Period samples: " << Np << endl;
+        cout << "Dispersion samples: " << Nd << endl;
+        line();
+    }
+}
+
+void SimulatedComplexSignal::generate_pulsar_signal_block(unsigned long dummy_repeat, unsigned long valid_repeat)
+{
+    if (data_type != "uint16") {
+        cout << "Warning: this function is intended for uint16 data type only." << endl;
+        return;
+    }
+    unsigned long long p[MSOSM_SIM_NPARAMS];
+    msosm::check(msosm_simulate_params(bw, dm, f0, period, dummy_repeat, valid_repeat, p),
+                 "generate_pulsar_signal_block");
+    range = (unsigned long)p[2];
+    signal_size = (unsigned long)p[4];
+    // whole chunks of 128 periods, as the reference allocates; the record is the first signal_size
+    const size_t alloc = (size_t)p[7] * 128 * Np;
+    free(signal_u16);
+    signal_u16 = (uint16_pair *)malloc(sizeof(uint16_pair) * alloc);
+    if (!signal_u16) msosm::check(MSOSM_ERR_ALLOC, "generate_pulsar_signal_block: host buffer");
+    memset(signal_u16 + signal_size, 0, sizeof(uint16_pair) * (alloc - signal_size));
+    msosm::check(msosm_simulate_pulsar_block(bw, dm, f0, period, dummy_repeat, valid_repeat, -1,
+                                             reinterpret_cast<uint32_t *>(signal_u16), nullptr, signal_size),
+                 "generate_pulsar_signal_block");
+}
+
+SimulatedComplexSignal::~SimulatedComplexSignal() { free(signal_u16); }

@@ -324,7 +324,8 @@ int msosm_simulate_noise(float bw, float dm, float f0, float period, unsigned lo
 /* the dispersion chirp e^{+i phi} at fftpoint bins (:166-198), interleaved complex64 */
 int msosm_simulate_chirp(float bw, float dm, float f0, unsigned long long fftpoint, float *H);
 /* the simulated record (:200-262): uint16 pairs (x + 32768 truncated as on x86, real in the low
- * half) and/or the complex64 series before the conversion, len >= repeat * Np; on `device` */
+ * half) and/or the complex64 series before the conversion, len >= repeat * Np; on `device`
+ * (-1: the current device) */
 int msosm_simulate_pulsar_block(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
                                 unsigned long valid_repeat, int device, uint32_t *out_u16, float *out_c64,
                                 unsigned long long len);

@@ -8,6 +8,7 @@
 //   uneven <bw> <f0> <push_blocks> <dms.txt> <in.c64> <out.c64>   per-DM lengths (check_msosm.cpp pattern)
 //   dada_info <file.dada> <n>                                   PSRDADA header and first samples (no GPU)
 //   psrdada <file.dada> <dm> <period> <count> <nbins>           msosm_psrdada.cpp:88-153 (dual-pol fold)
+//   simulate <bw> <dm> <f0> <period> <dummy> <valid> <out.u16>  check_msosm.cpp:128-133 (GPU simulator)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -21,6 +22,7 @@
 #include "psrdada.h"
 #include "msosm_gpu_batch.h"
 #include "osm_gpu_batch.h"
+#include "simulated_complex_signal.h"
 
 static std::vector<uint16_pair> read_u16(const std::string &path)
 {
@@ -47,6 +49,16 @@ int main(int argc, char **argv)
             std::cout << "s" << i << " " << file.pol1_in[i].first << " " << file.pol1_in[i].second << " "
                       << file.pol2_in[i].first << " " << file.pol2_in[i].second << std::endl;
         std::cout << "last " << file.pol1_in[2048].first << " " << file.pol2_in[2048].first << std::endl;
+        return 0;
+    }
+    if (mode == "simulate") {
+        SimulatedComplexSignal sim(std::stof(argv[2]), std::stof(argv[3]), std::stof(argv[4]), std::stof(argv[5]),
+                                   "uint16");
+        sim.generate_pulsar_signal_block(std::stoul(argv[6]), std::stoul(argv[7]));
+        std::ofstream of(argv[8], std::ios::binary);
+        of.write(reinterpret_cast<const char *>(sim.signal_u16), sim.signal_size * sizeof(uint16_pair));
+        std::cout << "Np " << sim.Np << " Nd " << sim.Nd << " range " << sim.range << " size " << sim.signal_size
+                  << std::endl;
         return 0;
     }
     if (mode == "psrdada") {

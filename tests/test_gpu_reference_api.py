@@ -175,3 +175,15 @@ def test_psrdada_dual_pol_fold(tmp_path, npol):
     err = np.max(np.abs(prof - ref) / np.abs(ref))
     print(f"  npol {npol}: profile max rel err {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_simulated_complex_signal_class(tmp_path):
+    # SimulatedComplexSignal(bw, dm, f0, period, "uint16").generate_pulsar_signal_block(dummy, valid), as
+    # check_msosm.cpp:128-133 calls it: the same record, bit for bit, as the C ABI's simulator
+    bw, dm, f0, period, dummy, valid = 16e6, 30.0, 1392e6, 1e-3, 2, 130
+    out = tmp_path / "sim.u16"
+    text = _run(["simulate", bw, dm, f0, period, dummy, valid, out], tmp_path)
+    p = ms.simulate_params(bw, dm, f0, period, dummy, valid)
+    assert f"Np {p.Np} Nd {p.Nd} range {p.range} size {p.samples}" in text
+    got = np.fromfile(out, dtype=np.uint32)
+    assert np.array_equal(got, ms.simulate_pulsar_block(bw, dm, f0, period, dummy, valid))
