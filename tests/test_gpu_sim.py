@@ -24,11 +24,13 @@ def codes_of(v):
     return one(v.real) | (one(v.imag) << np.uint32(16))
 
 
-@pytest.mark.parametrize("case", ["2^22 one workgroup", "2^24 six-step", "2^22 two chunks"])
+@pytest.mark.parametrize("case", ["2^22 one workgroup", "2^24 six-step", "2^22 two chunks", "2^26 harness band"])
 def test_simulator_matches_fp64_restatement(case):
     bw, dm, f0 = 16e6, 30.0, 1392e6
+    if case == "2^26 harness band":  # check_msosm.cpp / check_osm.cpp defaults: 128 MHz, DM 75, 1 GHz, 2.048 ms
+        bw, dm, f0 = 128e6, 75.0, 1e9
     period, dummy, valid = {"2^22 one workgroup": (1e-3, 0, 128), "2^24 six-step": (4e-3, 0, 128),
-                            "2^22 two chunks": (1e-3, 1, 200)}[case]
+                            "2^22 two chunks": (1e-3, 1, 200), "2^26 harness band": (0.002048, 0, 128)}[case]
     p = ms.simulate_params(bw, dm, f0, period, dummy, valid)
     print(f"case {case}: Np={p.Np} Nd={p.Nd} fftpoint=2^{p.fftpoint.bit_length() - 1} chunks={p.chunks}", flush=True)
     codes, series = ms.simulate_pulsar_block(bw, dm, f0, period, dummy, valid, series=True)
