@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
     ap.add_argument("--order", choices=ORDERS + [o + "+split" for o in ORDERS], default=None,
                     help="fixed unit order instead of the autotuner (profiling runs)")
-    ap.add_argument("--refine-steps", type=int, default=40)
+    ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--push-blocks", type=int, default=0,
                     help="uneven workload: blocks of the longest length per push (default 4)")
     ap.add_argument("--graph", action="store_true", help="uneven workload: run each one-launch push as a hipGraph")
