@@ -63,7 +63,7 @@ def host_threads():
     return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
 
 
-def cpu_baseline(design, n_samples=1 << 25):
+def cpu_baseline(design, n_samples=3 << 24):
     """FP32 CPU port of the same MS-OSM + fold (oracle/cpu_msosm.c: FFTW3-API transforms from Intel MKL,
     OpenMP over DM trials on every host thread) on a bounded sample: all 512 trials of the grid over the
     first n_samples samples of a uniform int8 input, dedispersed and folded to the same 1024 bins."""
@@ -81,8 +81,10 @@ def cpu_baseline(design, n_samples=1 << 25):
                        nthreads=threads, call_size=W["batch"] * design.M)
     dt = time.perf_counter() - t0
     n_dm = len(design.dms)
+    tz = (n_samples & -n_samples).bit_length() - 1  # n_samples = odd x 2^tz
+    size = f"2^{tz}" if n_samples == 1 << tz else f"{n_samples >> tz} x 2^{tz}"
     return {"value": n_dm * n_samples / dt, "unit": "DM-trial*samples/s", "cores": threads, "kind": "port",
-            "sample": f"all {n_dm} DM trials x {n_samples} samples (2^{n_samples.bit_length() - 1}), FFT length "
+            "sample": f"all {n_dm} DM trials x {n_samples} samples ({size}), FFT length "
                       f"2^{design.M.bit_length()}, FP32 MS-OSM + fold in {dt:.1f} s on {threads} threads "
                       f"(OpenMP over trials); transforms: {orc.cpu_backend()}"}
 
