@@ -286,6 +286,10 @@ def bench_uneven(args):
     roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": alg / avg_s / HBM_PEAK, "traffic": None, "kernel": "inv_multi_kernel",
             "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "alg_bytes_per_launch": alg}
+    tpath = os.path.join(ROOT, "profiles", "traffic_inv_multi_kernel_msosm_uneven_c64.json")
+    if os.path.exists(tpath) and not args.graph:
+        with open(tpath) as fh:
+            roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
     out = {
         "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 2, "ms_per_step": one["ms_per_push"] * pushes, "higher_is_better": True,
