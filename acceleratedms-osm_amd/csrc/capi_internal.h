@@ -1,6 +1,7 @@
 // Internal declarations shared by the C-ABI implementation files.
 #pragma once
 
+#include <complex>
 #include <string>
 
 namespace msosm {
@@ -19,6 +20,8 @@ int design_transfer_functions(float bw, float f0, const float *dms, int num_dms,
                               int nthreads, int deriv = 0);
 const Fft64Handle *design_fft_create(int M);
 void design_fft_destroy(const Fft64Handle *h);
+// unnormalised FP64 transform in place, sign -1: e^{-i}, +1: e^{+i}
+void design_fft_run(const Fft64Handle *h, std::complex<double> *x, int sign);
 // e^{+2 pi i k/n} in FP64 (octant-reduced)
 void unit_root(long k, long n, double *re, double *im);
 

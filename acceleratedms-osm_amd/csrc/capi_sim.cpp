@@ -78,6 +78,84 @@ void set_tw(SimFftArgs &a, const SimTw &t)
     a.tw_s = t.s;
 }
 
+// one complex FFT of 2^lg points on the device: one workgroup up to 2^22 points, above that the
+// six-step form N = A B over two in-workgroup lengths (n = c + B a, k = a' + A c': columns of length A
+// at stride B times the twiddle W_N^{a' c}, then rows of length B written transposed in natural order)
+struct SimPlan {
+    int lg = 0, lga = 0, lgb = 0, grid = 1;
+    bool one = true;
+    long long N = 0, A = 0, B = 0;
+    SimTw ta, tb;
+    float2 *scr = nullptr, *tmp = nullptr;
+};
+
+bool sim_plan_supported(int lg)
+{
+    return sim_length_supported(lg) || (sim_length_supported((lg + 1) / 2) && sim_length_supported(lg / 2));
+}
+
+hipError_t make_plan(SimDev &d, int lg, int cus, SimPlan *p)
+{
+    p->lg = lg;
+    p->one = sim_length_supported(lg);
+    p->lga = (lg + 1) / 2;
+    p->lgb = lg / 2;
+    p->N = 1LL << lg;
+    p->A = 1LL << p->lga;
+    p->B = 1LL << p->lgb;
+    p->grid = p->one ? 1 : std::max(8, cus);
+    const long long Lsub = p->one ? p->N : p->A; // A >= B
+    hipError_t e = d.alloc(reinterpret_cast<void **>(&p->scr), (size_t)p->grid * Lsub * sizeof(float2));
+    if (e == hipSuccess && !p->one) e = d.alloc(reinterpret_cast<void **>(&p->tmp), p->N * sizeof(float2));
+    if (e == hipSuccess) e = make_tw(d, p->one ? lg : p->lga, &p->ta);
+    if (e == hipSuccess && !p->one) e = make_tw(d, p->lgb, &p->tb);
+    return e;
+}
+
+// dst <- the transform of src (direction dir) through the output stage `mode`; `a` carries the
+// stage's chirp or extraction fields. With mode SIM_EXTRACT, dst is unused.
+hipError_t run_plan(const SimPlan &p, const float2 *src, float2 *dst, int dir, int mode, SimFftArgs a, hipStream_t st)
+{
+    a.scratch = p.scr;
+    a.big_n = p.N;
+    if (p.one) {
+        set_tw(a, p.ta);
+        a.ntrans = 1;
+        a.in_t = a.out_t = 0;
+        a.in_e = a.out_e = 1;
+        a.src = src;
+        a.dst = dst;
+        a.mode = mode;
+        return launch_sim_fft(p.lg, dir, a, p.grid, st);
+    }
+    set_tw(a, p.ta); // columns
+    a.ntrans = p.B;
+    a.in_t = a.out_t = 1;
+    a.in_e = a.out_e = p.B;
+    a.src = src;
+    a.dst = p.tmp;
+    a.mode = SIM_TWIDDLE;
+    hipError_t e = launch_sim_fft(p.lga, dir, a, p.grid, st);
+    if (e != hipSuccess) return e;
+    set_tw(a, p.tb); // rows
+    a.ntrans = p.A;
+    a.in_t = p.B;
+    a.in_e = 1;
+    a.out_t = 1;
+    a.out_e = p.A;
+    a.src = p.tmp;
+    a.dst = dst;
+    a.mode = mode;
+    return launch_sim_fft(p.lgb, dir, a, p.grid, st);
+}
+
+int log2_of(unsigned long n)
+{
+    int lg = 0;
+    while ((1UL << lg) < n) ++lg;
+    return lg;
+}
+
 int host_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
 
 } // namespace
@@ -114,7 +192,7 @@ extern "C" int msosm_simulate_noise(float bw, float dm, float f0, float period, 
 
 extern "C" int msosm_simulate_chirp(float bw, float dm, float f0, unsigned long long fftpoint, float *H)
 {
-    if (!H || fftpoint < 2 || (fftpoint & (fftpoint - 1))) return set_error(MSOSM_ERR_ARG, "simulate_chirp: bad length");
+    if (!H || fftpoint == 0 || (fftpoint & (fftpoint - 1))) return set_error(MSOSM_ERR_ARG, "simulate_chirp: bad length");
     sim_chirp(bw, dm, f0, (unsigned long)fftpoint, H, host_threads());
     return MSOSM_OK;
 }
@@ -129,15 +207,10 @@ extern "C" int msosm_simulate_pulsar_block(float bw, float dm, float f0, float p
     if ((!out_u16 && !out_c64) || len < p.signal_size)
         return set_error(MSOSM_ERR_ARG, "simulate: output buffers of repeat * Np samples needed (" +
                                             std::to_string(p.signal_size) + ")");
-    int lg = 0;
-    while ((1UL << lg) < p.fftpoint) ++lg;
-    // one workgroup per transform up to 2^22; the six-step form over two supported lengths above
-    const bool one = sim_length_supported(lg);
-    const int lga = (lg + 1) / 2, lgb = lg / 2;
-    if (!one && !(sim_length_supported(lga) && sim_length_supported(lgb)))
+    const int lg = log2_of(p.fftpoint);
+    if (!sim_plan_supported(lg))
         return set_error(MSOSM_ERR_UNSUPPORTED, "simulate: FFT length 2^" + std::to_string(lg) +
                                                     " (supported: 2^12..2^22 and 2^24..2^44)");
-    const long long N = (long long)p.fftpoint, A = 1LL << lga, B = 1LL << lgb;
 
     // host: noise periods and chirp, exactly as the reference builds them
     std::vector<float> periods(2 * (size_t)p.periods_len), H(2 * (size_t)p.fftpoint);
@@ -150,80 +223,153 @@ extern "C" int msosm_simulate_pulsar_block(float bw, float dm, float f0, float p
     SIMCHK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));
     int cus = 0;
     SIMCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    float2 *dper = nullptr, *dH = nullptr, *dX = nullptr, *dT = nullptr, *scr = nullptr, *dc64 = nullptr;
+    const long long N = (long long)p.fftpoint;
+    float2 *dper = nullptr, *dH = nullptr, *dX = nullptr, *dc64 = nullptr;
     uint32_t *du16 = nullptr;
     SIMCHK(d.alloc(reinterpret_cast<void **>(&dper), p.periods_len * sizeof(float2)));
     SIMCHK(d.alloc(reinterpret_cast<void **>(&dH), N * sizeof(float2)));
     SIMCHK(d.alloc(reinterpret_cast<void **>(&dX), N * sizeof(float2)));
-    if (!one) SIMCHK(d.alloc(reinterpret_cast<void **>(&dT), N * sizeof(float2)));
     if (out_c64) SIMCHK(d.alloc(reinterpret_cast<void **>(&dc64), p.signal_size * sizeof(float2)));
     if (out_u16) SIMCHK(d.alloc(reinterpret_cast<void **>(&du16), p.signal_size * sizeof(uint32_t)));
-    const long long Lsub = one ? N : A; // A >= B
-    const int grid = one ? 1 : std::max(8, cus);
-    SIMCHK(d.alloc(reinterpret_cast<void **>(&scr), (size_t)grid * Lsub * sizeof(float2)));
+    SimPlan plan;
+    SIMCHK(make_plan(d, lg, cus, &plan));
     SIMCHK(hipMemcpyAsync(dper, periods.data(), p.periods_len * sizeof(float2), hipMemcpyHostToDevice, d.st));
     SIMCHK(hipMemcpyAsync(dH, H.data(), N * sizeof(float2), hipMemcpyHostToDevice, d.st));
-    SimTw ta, tb;
-    SIMCHK(make_tw(d, one ? lg : lga, &ta));
-    if (!one) SIMCHK(make_tw(d, lgb, &tb));
 
     const long long per_chunk = (long long)SIM_PERIODS_PER_CHUNK * p.Np;
     for (unsigned long r = 0; r < p.mul_repeat; ++r) {
-        const float2 *window = dper + r * per_chunk;
+        // forward transform times the chirp, then the inverse with the extraction
         const long long done = (long long)r * per_chunk;
         SimFftArgs a{};
-        a.scratch = scr;
+        a.chirp = dH;
+        SIMCHK(run_plan(plan, dper + done, dX, -1, SIM_CHIRP, a, d.st));
         a.ex_start = (long long)p.start;
         a.ex_count = std::min<long long>(per_chunk, (long long)p.signal_size - done);
         a.ex_scale = 1.0f / static_cast<float>(p.fftpoint);
         a.ex_c64 = dc64 ? dc64 + done : nullptr;
         a.ex_u16 = du16 ? du16 + done : nullptr;
-        a.chirp = dH;
-        a.big_n = N;
-        if (one) {
-            // forward transform times the chirp, then the inverse with the extraction
-            set_tw(a, ta);
-            a.ntrans = 1;
-            a.in_t = a.out_t = 0;
-            a.in_e = a.out_e = 1;
-            a.src = window;
-            a.dst = dX;
-            a.mode = SIM_CHIRP;
-            SIMCHK(launch_sim_fft(lg, -1, a, grid, d.st));
-            a.src = dX;
-            a.mode = SIM_EXTRACT;
-            SIMCHK(launch_sim_fft(lg, +1, a, grid, d.st));
-        } else {
-            // six-step: N = A B, n = c + B a, k = a' + A c'. Columns (length A, stride B) with the
-            // twiddle W_N^{a' c}, then rows (length B) written transposed in natural order.
-            auto cols = [&](const float2 *src, float2 *dst, int dir) {
-                set_tw(a, ta);
-                a.ntrans = B;
-                a.in_t = a.out_t = 1;
-                a.in_e = a.out_e = B;
-                a.src = src;
-                a.dst = dst;
-                a.mode = SIM_TWIDDLE;
-                return launch_sim_fft(lga, dir, a, grid, d.st);
-            };
-            auto rows = [&](const float2 *src, float2 *dst, int dir, int mode) {
-                set_tw(a, tb);
-                a.ntrans = A;
-                a.in_t = B;
-                a.in_e = 1;
-                a.out_t = 1;
-                a.out_e = A;
-                a.src = src;
-                a.dst = dst;
-                a.mode = mode;
-                return launch_sim_fft(lgb, dir, a, grid, d.st);
-            };
-            SIMCHK(cols(window, dT, -1));
-            SIMCHK(rows(dT, dX, -1, SIM_CHIRP));
-            SIMCHK(cols(dX, dT, +1));
-            SIMCHK(rows(dT, nullptr, +1, SIM_EXTRACT));
-        }
+        SIMCHK(run_plan(plan, dX, nullptr, +1, SIM_EXTRACT, a, d.st));
     }
+    if (out_c64) SIMCHK(hipMemcpyAsync(out_c64, dc64, p.signal_size * sizeof(float2), hipMemcpyDeviceToHost, d.st));
+    if (out_u16) SIMCHK(hipMemcpyAsync(out_u16, du16, p.signal_size * sizeof(uint32_t), hipMemcpyDeviceToHost, d.st));
+    SIMCHK(hipStreamSynchronize(d.st));
+    return MSOSM_OK;
+}
+
+// ---- generate_pulsar_signal_new (simulated_complex_signal.cpp:279-467): the whole record of
+// repeat + 2 guard periods dispersed by one FFT through a causal impulse response
+
+extern "C" int msosm_simulate_params_new(float bw, float dm, float f0, float period, unsigned long repeat,
+                                         unsigned long long *out)
+{
+    SimParamsNew p;
+    if (!out) return set_error(MSOSM_ERR_ARG, "simulate_params_new: null output");
+    if (sim_params_new(bw, dm, f0, period, repeat, &p) != 0)
+        return set_error(MSOSM_ERR_ARG, "simulate_params_new: bw, f0, period > 0, dm >= 0 and at least one period");
+    const unsigned long long v[MSOSM_SIM_NPARAMS] = {p.Np,          p.Nd,           p.range, p.fftpoint, p.signal_size,
+                                                     p.filter_point, p.total_periods, p.start};
+    std::copy(v, v + MSOSM_SIM_NPARAMS, out);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_simulate_noise_new(float bw, float dm, float f0, float period, unsigned long repeat,
+                                        float *record, unsigned long long len)
+{
+    SimParamsNew p;
+    if (sim_params_new(bw, dm, f0, period, repeat, &p) != 0)
+        return set_error(MSOSM_ERR_ARG, "simulate_noise_new: bw, f0, period > 0, dm >= 0 and at least one period");
+    if (!record || len < p.fftpoint) return set_error(MSOSM_ERR_ARG, "simulate_noise_new: buffer too small");
+    sim_noise_new(p, record);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_simulate_pulsar_new(float bw, float dm, float f0, float period, unsigned long repeat, int device,
+                                         uint32_t *out_u16, float *out_c64, unsigned long long len)
+{
+    SimParamsNew p;
+    if (sim_params_new(bw, dm, f0, period, repeat, &p) != 0)
+        return set_error(MSOSM_ERR_ARG, "simulate_new: bw, f0, period > 0, dm >= 0 and at least one period");
+    if ((!out_u16 && !out_c64) || len < p.signal_size)
+        return set_error(MSOSM_ERR_ARG, "simulate_new: output buffers of repeat * Np samples needed (" +
+                                            std::to_string(p.signal_size) + ")");
+    const int lg = log2_of(p.fftpoint), lgf = log2_of(p.filter_point);
+    if (!sim_plan_supported(lg))
+        return set_error(MSOSM_ERR_UNSUPPORTED, "simulate_new: FFT length 2^" + std::to_string(lg) +
+                                                    " (supported: 2^12..2^22 and 2^24..2^44)");
+    // the filter design's inverse FFT runs on the device where its length is supported; shorter
+    // filters and 2^23 points run in FP64 on the host (filter_point <= fftpoint, so nothing else)
+    const bool dev_filter = sim_plan_supported(lgf);
+    const long long N = (long long)p.fftpoint, F = (long long)p.filter_point, Nd = (long long)p.Nd;
+
+    // host: the chirp on the filter grid (:338-366) and the seed-1 record (:330-336)
+    std::vector<float> Hf(2 * (size_t)F), record(2 * (size_t)N);
+    sim_chirp(bw, dm, f0, p.filter_point, Hf.data(), host_threads());
+    sim_noise_new(p, record.data());
+    std::vector<float2> h_host;
+    if (!dev_filter) {
+        // :368-379: h = IFFT(H_filter) / filter_point
+        std::vector<std::complex<double>> x(F);
+        for (long long i = 0; i < F; ++i) x[i] = {Hf[2 * i], Hf[2 * i + 1]};
+        const Fft64Handle *fft = design_fft_create((int)F);
+        design_fft_run(fft, x.data(), +1);
+        design_fft_destroy(fft);
+        const float inv = 1.0f / static_cast<float>(F);
+        h_host.resize(F);
+        for (long long i = 0; i < F; ++i)
+            h_host[i] = make_float2((float)x[i].real() * inv, (float)x[i].imag() * inv);
+    }
+
+    if (device < 0) SIMCHK(hipGetDevice(&device)); // the caller's current device
+    SIMCHK(hipSetDevice(device));
+    SimDev d;
+    SIMCHK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));
+    int cus = 0;
+    SIMCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    float2 *drec = nullptr, *dtap = nullptr, *dH = nullptr, *dh = nullptr, *dc64 = nullptr;
+    uint32_t *du16 = nullptr;
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&drec), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dtap), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dH), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dh), F * sizeof(float2)));
+    if (out_c64) SIMCHK(d.alloc(reinterpret_cast<void **>(&dc64), p.signal_size * sizeof(float2)));
+    if (out_u16) SIMCHK(d.alloc(reinterpret_cast<void **>(&du16), p.signal_size * sizeof(uint32_t)));
+    SimPlan plan;
+    SIMCHK(make_plan(d, lg, cus, &plan));
+    SIMCHK(hipMemcpyAsync(drec, record.data(), N * sizeof(float2), hipMemcpyHostToDevice, d.st));
+
+    if (dev_filter) {
+        // h = IFFT(H_filter) times 1/filter_point, through the extraction stage into dh
+        SimPlan fplan;
+        SIMCHK(make_plan(d, lgf, cus, &fplan));
+        float2 *dHf = nullptr;
+        SIMCHK(d.alloc(reinterpret_cast<void **>(&dHf), F * sizeof(float2)));
+        SIMCHK(hipMemcpyAsync(dHf, Hf.data(), F * sizeof(float2), hipMemcpyHostToDevice, d.st));
+        SimFftArgs a{};
+        a.ex_start = 0;
+        a.ex_count = F;
+        a.ex_scale = 1.0f / static_cast<float>(F);
+        a.ex_c64 = dh;
+        SIMCHK(run_plan(fplan, dHf, nullptr, +1, SIM_EXTRACT, a, d.st));
+    } else {
+        SIMCHK(hipMemcpyAsync(dh, h_host.data(), F * sizeof(float2), hipMemcpyHostToDevice, d.st));
+    }
+    // :381-396: the causal response, the last Nd taps then h[0] at index Nd, zero-padded to fftpoint
+    // (filter_point >= Nd + 1 and fftpoint >= Nd + 1, so both copies stay inside their buffers)
+    SIMCHK(hipMemsetAsync(dtap, 0, N * sizeof(float2), d.st));
+    if (Nd > 0) SIMCHK(hipMemcpyAsync(dtap, dh + (F - Nd), Nd * sizeof(float2), hipMemcpyDeviceToDevice, d.st));
+    SIMCHK(hipMemcpyAsync(dtap + Nd, dh, sizeof(float2), hipMemcpyDeviceToDevice, d.st));
+    SimFftArgs a{};
+    SIMCHK(run_plan(plan, dtap, dH, -1, SIM_PLAIN, a, d.st)); // :398-400
+    // :409-427: the record's spectrum times the response's, back, times 1/fftpoint; :431-455: the
+    // repeat periods from guard * Np + Nd, and their uint16 pairs
+    a.chirp = dH;
+    SIMCHK(run_plan(plan, drec, dtap, -1, SIM_CHIRP, a, d.st));
+    a.ex_start = (long long)p.start;
+    a.ex_count = (long long)p.signal_size;
+    a.ex_scale = 1.0f / static_cast<float>(p.fftpoint);
+    a.ex_c64 = dc64;
+    a.ex_u16 = du16;
+    SIMCHK(run_plan(plan, dtap, nullptr, +1, SIM_EXTRACT, a, d.st));
     if (out_c64) SIMCHK(hipMemcpyAsync(out_c64, dc64, p.signal_size * sizeof(float2), hipMemcpyDeviceToHost, d.st));
     if (out_u16) SIMCHK(hipMemcpyAsync(out_u16, du16, p.signal_size * sizeof(uint32_t), hipMemcpyDeviceToHost, d.st));
     SIMCHK(hipStreamSynchronize(d.st));

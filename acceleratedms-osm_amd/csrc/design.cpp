@@ -253,6 +253,10 @@ void unit_root(long k, long n, double *re, double *im)
 
 const Fft64Handle *design_fft_create(int M) { return reinterpret_cast<const Fft64Handle *>(new Fft64(M)); }
 void design_fft_destroy(const Fft64Handle *h) { delete reinterpret_cast<const Fft64 *>(h); }
+void design_fft_run(const Fft64Handle *h, std::complex<double> *x, int sign)
+{
+    reinterpret_cast<const Fft64 *>(h)->run(x, sign);
+}
 
 int design_transfer_functions(float bw, float f0, const float *dms, int num_dms, int M, int phase_mode, float *H,
                               int nthreads, int deriv)

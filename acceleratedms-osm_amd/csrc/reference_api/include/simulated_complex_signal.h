@@ -1,5 +1,5 @@
 // Reference-compatible pulsar simulator class (simulated_pulsar/include/simulated_complex_signal.h:8-55)
-// over the C ABI: generate_pulsar_signal_block disperses on the GPU (msosm_simulate_pulsar_block).
+// over the C ABI: both generators disperse on the GPU (msosm_simulate_pulsar_block, _new).
 #pragma once
 
 #include <string>
@@ -11,10 +11,12 @@ public:
     SimulatedComplexSignal(float bw, float dm, float f0, float period, string mode = "complex");
     // simulated_complex_signal.cpp:74-277; the record is dummy_repeat + valid_repeat periods
     void generate_pulsar_signal_block(unsigned long dummy_repeat = 0, unsigned long valid_repeat = 1);
+    // simulated_complex_signal.cpp:279-467; fills signal, and signal_u16 in "uint16" mode
+    void generate_pulsar_signal_new(unsigned long repeat = 1);
     ~SimulatedComplexSignal();
 
 public:
-    // the simulated signal: only the uint16 form is produced (as by the block generator)
+    // the simulated signal: the block generator fills signal_u16 only, as the reference's does
     fftwf_complex *signal = nullptr;
     uint16_pair *signal_u16 = nullptr;
     string data_type;

@@ -1,6 +1,7 @@
 // Reference-compatible SimulatedComplexSignal (simulated_pulsar/simulated_complex_signal.cpp:16-41,
-// 74-277): the constructor sets the reference's members; generate_pulsar_signal_block returns the same
-// record (noise and chirp as the reference builds them, dispersion on the current GPU).
+// 74-277, 279-467): the constructor sets the reference's members; generate_pulsar_signal_block and
+// generate_pulsar_signal_new return the same records (noise and chirp as the reference builds them,
+// dispersion on the current GPU).
 #include "simulated_complex_signal.h"
 
 #include <cstdlib>
@@ -48,4 +49,26 @@ void SimulatedComplexSignal::generate_pulsar_signal_block(unsigned long dummy_re
                  "generate_pulsar_signal_block");
 }
 
-SimulatedComplexSignal::~SimulatedComplexSignal() { free(signal_u16); }
+void SimulatedComplexSignal::generate_pulsar_signal_new(unsigned long repeat)
+{
+    unsigned long long p[MSOSM_SIM_NPARAMS];
+    msosm::check(msosm_simulate_params_new(bw, dm, f0, period, repeat, p), "generate_pulsar_signal_new");
+    range = (unsigned long)p[2];
+    signal_size = (unsigned long)p[4];
+    free(signal);
+    free(signal_u16);
+    signal_u16 = nullptr;
+    signal = (fftwf_complex *)malloc(sizeof(fftwf_complex) * signal_size);
+    if (data_type == "uint16") signal_u16 = (uint16_pair *)malloc(sizeof(uint16_pair) * signal_size);
+    if (!signal || (data_type == "uint16" && !signal_u16))
+        msosm::check(MSOSM_ERR_ALLOC, "generate_pulsar_signal_new: host buffer");
+    msosm::check(msosm_simulate_pulsar_new(bw, dm, f0, period, repeat, -1, reinterpret_cast<uint32_t *>(signal_u16),
+                                           reinterpret_cast<float *>(signal), signal_size),
+                 "generate_pulsar_signal_new");
+}
+
+SimulatedComplexSignal::~SimulatedComplexSignal()
+{
+    free(signal);
+    free(signal_u16);
+}

@@ -57,14 +57,13 @@ static void test_pulse(float *sig, const std::vector<double> &amp, std::mt19937 
     }
 }
 
-void sim_noise(const SimParams &p, float *periods)
+// the three-Gaussian profile's per-component amplitude (:47-63), the same for every period
+static std::vector<double> pulse_amp(unsigned long Np)
 {
-    std::fill(periods, periods + 2 * p.periods_len, 0.f);
-    // the three-Gaussian profile and its scale are the same for every period
-    std::vector<double> profile(p.Np), amp(p.Np);
+    std::vector<double> profile(Np), amp(Np);
     double max_amp = 0.0;
-    for (unsigned long i = 0; i < p.Np; i++) {
-        const double x = static_cast<double>(i) / static_cast<double>(p.Np);
+    for (unsigned long i = 0; i < Np; i++) {
+        const double x = static_cast<double>(i) / static_cast<double>(Np);
         const double p1 = 0.45 * std::exp(-0.5 * std::pow((x - 0.455) / 0.008, 2.0));
         const double p2 = 1.00 * std::exp(-0.5 * std::pow((x - 0.500) / 0.012, 2.0));
         const double p3 = 0.28 * std::exp(-0.5 * std::pow((x - 0.555) / 0.006, 2.0));
@@ -72,7 +71,45 @@ void sim_noise(const SimParams &p, float *periods)
         max_amp = std::max(max_amp, profile[i]);
     }
     const double scale = 10000.0 / max_amp;
-    for (unsigned long i = 0; i < p.Np; i++) amp[i] = profile[i] * scale / std::sqrt(2.0);
+    for (unsigned long i = 0; i < Np; i++) amp[i] = profile[i] * scale / std::sqrt(2.0);
+    return amp;
+}
+
+int sim_params_new(float bw, float dm, float f0, float period, unsigned long repeat, SimParamsNew *p)
+{
+    SimParams b; // the constructor's Np and Nd
+    if (repeat == 0 || sim_params(bw, dm, f0, period, 0, repeat, &b) != 0) return -1;
+    // generate_pulsar_signal_new (:279-328)
+    p->Np = b.Np;
+    p->Nd = b.Nd;
+    p->repeat = repeat;
+    p->signal_size = repeat * b.Np;
+    p->range = (unsigned long)std::ceil((double)b.Nd / (double)b.Np);
+    p->guard = p->range + 2;
+    p->total_periods = repeat + 2 * p->guard;
+    const unsigned long required = p->total_periods * b.Np + b.Nd;
+    p->fftpoint = 1;
+    while (p->fftpoint < required) p->fftpoint <<= 1;
+    // :343-345
+    p->filter_point = 1;
+    while (p->filter_point < b.Nd + 1) p->filter_point <<= 1;
+    p->start = p->guard * b.Np + b.Nd; // :431
+    return 0;
+}
+
+void sim_noise_new(const SimParamsNew &p, float *record)
+{
+    // :330-336: every period from one seed-1 stream, the rest of the FFT window zero
+    std::fill(record + 2 * p.total_periods * p.Np, record + 2 * p.fftpoint, 0.f);
+    const std::vector<double> amp = pulse_amp(p.Np);
+    std::mt19937 gen(1);
+    for (unsigned long q = 0; q < p.total_periods; q++) test_pulse(record + 2 * q * p.Np, amp, gen);
+}
+
+void sim_noise(const SimParams &p, float *periods)
+{
+    std::fill(periods, periods + 2 * p.periods_len, 0.f);
+    const std::vector<double> amp = pulse_amp(p.Np);
     // periods: dummy0 from seed 1, then valid and dummy1 from seed 2 (:119-161, no cached file)
     std::mt19937 gen(1);
     unsigned long q = 0;

@@ -15,9 +15,22 @@ struct SimParams {
     unsigned long start;                // first sample of the extracted part of a chunk's window
 };
 
+// generate_pulsar_signal_new: one continuous record, dispersed by one FFT through a causal filter
+struct SimParamsNew {
+    unsigned long Np, Nd, range, guard; // as above
+    unsigned long repeat, signal_size;  // periods returned and their samples (repeat * Np)
+    unsigned long total_periods;        // repeat + 2 guard periods generated (all from seed 1)
+    unsigned long fftpoint;             // record FFT length >= total_periods * Np + Nd
+    unsigned long filter_point;         // filter design length >= Nd + 1
+    unsigned long start;                // first extracted sample: guard * Np + Nd
+};
+
 int sim_params(float bw, float dm, float f0, float period, unsigned long dummy_repeat, unsigned long valid_repeat,
                SimParams *p);
 void sim_noise(const SimParams &p, float *periods);
+int sim_params_new(float bw, float dm, float f0, float period, unsigned long repeat, SimParamsNew *p);
+// the record: total_periods periods from one seed-1 stream, zero-padded to fftpoint samples
+void sim_noise_new(const SimParamsNew &p, float *record);
 void sim_chirp(float bw, float dm, float f0, unsigned long fftpoint, float *H, int nthreads);
 
 } // namespace msosm

@@ -45,6 +45,7 @@ EXPORTS = [
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
     "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune", "msosm_stream_wait",
     "msosm_simulate_params", "msosm_simulate_noise", "msosm_simulate_chirp", "msosm_simulate_pulsar_block",
+    "msosm_simulate_params_new", "msosm_simulate_noise_new", "msosm_simulate_pulsar_new",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
@@ -162,6 +163,12 @@ def lib():
         "msosm_simulate_noise": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong, fp,
                                   C.c_ulonglong], C.c_int),
         "msosm_simulate_chirp": ([C.c_float, C.c_float, C.c_float, C.c_ulonglong, fp], C.c_int),
+        "msosm_simulate_params_new": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong,
+                                       C.POINTER(C.c_ulonglong)], C.c_int),
+        "msosm_simulate_noise_new": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, fp, C.c_ulonglong],
+                                     C.c_int),
+        "msosm_simulate_pulsar_new": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_int,
+                                       C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
         "msosm_simulate_pulsar_block": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong, C.c_int,
                                          C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
     }
@@ -665,3 +672,41 @@ def simulate_pulsar_block(bw: float, dm: float, f0: float, period: float, dummy_
                                              _fp(c64.view(np.float32)) if series else None, p.samples))
     return (codes, c64) if series else codes
 
+
+class SimParamsNew(NamedTuple):
+    Np: int             # samples per period
+    Nd: int             # dispersion samples
+    range: int          # ceil(Nd / Np)
+    fftpoint: int       # record FFT length
+    samples: int        # samples returned: repeat * Np
+    filter_point: int   # filter design length
+    periods: int        # periods generated: repeat + 2 (range + 2)
+    start: int          # first extracted sample: (range + 2) Np + Nd
+
+
+def simulate_params_new(bw: float, dm: float, f0: float, period: float, repeat: int = 1) -> SimParamsNew:
+    """Sizes of generate_pulsar_signal_new (simulated_complex_signal.cpp:279-467)."""
+    v = (C.c_ulonglong * 8)()
+    _check(lib().msosm_simulate_params_new(bw, dm, f0, period, repeat, v))
+    return SimParamsNew(*[int(x) for x in v])
+
+
+def simulate_noise_new(bw: float, dm: float, f0: float, period: float, repeat: int = 1) -> np.ndarray:
+    """The undispersed seed-1 record, zero-padded to fftpoint samples (complex64)."""
+    p = simulate_params_new(bw, dm, f0, period, repeat)
+    out = np.empty(p.fftpoint, dtype=np.complex64)
+    _check(lib().msosm_simulate_noise_new(bw, dm, f0, period, repeat, _fp(out.view(np.float32)), p.fftpoint))
+    return out
+
+
+def simulate_pulsar_new(bw: float, dm: float, f0: float, period: float, repeat: int = 1, device: int = 0,
+                        series: bool = False):
+    """generate_pulsar_signal_new's record as uint16 pairs (uint32, real in the low half); with
+    series=True also the complex64 series before the conversion."""
+    p = simulate_params_new(bw, dm, f0, period, repeat)
+    codes = np.empty(p.samples, dtype=np.uint32)
+    c64 = np.empty(p.samples, dtype=np.complex64) if series else None
+    _check(lib().msosm_simulate_pulsar_new(bw, dm, f0, period, repeat, device,
+                                           codes.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           _fp(c64.view(np.float32)) if series else None, p.samples))
+    return (codes, c64) if series else codes

@@ -329,6 +329,22 @@ int msosm_simulate_chirp(float bw, float dm, float f0, unsigned long long fftpoi
 int msosm_simulate_pulsar_block(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
                                 unsigned long valid_repeat, int device, uint32_t *out_u16, float *out_c64,
                                 unsigned long long len);
+/* SimulatedComplexSignal::generate_pulsar_signal_new (simulated_complex_signal.cpp:279-467): repeat
+ * periods cut from one record of repeat + 2 (range + 2) periods drawn from one seed-1 stream, dispersed
+ * by one FFT of fftpoint >= that record + Nd points through the causal response (the inverse FFT of the
+ * chirp on filter_point >= Nd + 1 bins, its last Nd taps then tap 0), and read from
+ * (range + 2) Np + Nd on. out[8] = {Np, Nd, range, fftpoint, samples returned, filter_point,
+ * periods generated, first extracted sample} (:284-345, 431) */
+int msosm_simulate_params_new(float bw, float dm, float f0, float period, unsigned long repeat,
+                              unsigned long long *out);
+/* the zero-padded record (generate_test_pulse over every period, :330-336), len >= fftpoint */
+int msosm_simulate_noise_new(float bw, float dm, float f0, float period, unsigned long repeat, float *record,
+                             unsigned long long len);
+/* the simulated record as in msosm_simulate_pulsar_block, len >= repeat * Np. Deviation: filters whose
+ * design length is below 2^12 or is 2^23 are designed with an FP64 inverse FFT on the host (the
+ * reference's is FP32); the other transforms are FP32 on the device, as the reference's are FP32. */
+int msosm_simulate_pulsar_new(float bw, float dm, float f0, float period, unsigned long repeat, int device,
+                              uint32_t *out_u16, float *out_c64, unsigned long long len);
 
 #ifdef __cplusplus
 }

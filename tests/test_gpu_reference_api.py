@@ -187,3 +187,16 @@ def test_simulated_complex_signal_class(tmp_path):
     assert f"Np {p.Np} Nd {p.Nd} range {p.range} size {p.samples}" in text
     got = np.fromfile(out, dtype=np.uint32)
     assert np.array_equal(got, ms.simulate_pulsar_block(bw, dm, f0, period, dummy, valid))
+
+
+def test_simulated_complex_signal_new(tmp_path):
+    # generate_pulsar_signal_new(repeat) in "uint16" mode fills both signal and signal_u16
+    bw, dm, f0, period, repeat = 16e6, 30.0, 1392e6, 1e-3, 12
+    out, outc = tmp_path / "sim.u16", tmp_path / "sim.c64"
+    text = _run(["simulate_new", bw, dm, f0, period, repeat, out, outc], tmp_path)
+    p = ms.simulate_params_new(bw, dm, f0, period, repeat)
+    assert f"Np {p.Np} Nd {p.Nd} range {p.range} size {p.samples}" in text
+    codes, series = ms.simulate_pulsar_new(bw, dm, f0, period, repeat, series=True)
+    assert np.array_equal(np.fromfile(out, dtype=np.uint32), codes)
+    assert np.array_equal(np.fromfile(outc, dtype=np.complex64), series)
+

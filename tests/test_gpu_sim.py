@@ -1,4 +1,4 @@
-"""GPU pulsar simulator (simulated_complex_signal.cpp:74-277) against an FP64 restatement.
+"""GPU pulsar simulator (simulated_complex_signal.cpp:74-277, 279-467) against FP64 restatements.
 
 The device disperses each chunk of 128 periods with one complex FFT of `fftpoint` points: one
 workgroup per transform up to 2^22 points, a six-step decomposition over two in-workgroup lengths
@@ -8,6 +8,11 @@ series within rel-L2 1e-5 (FP32 transforms against FP64); the uint16 pairs are e
 reference's conversion of the returned series, and differ from the FP64 series' codes by at most one
 step where the two roundings straddle an integer. End to end, dedispersing the simulated uint16
 record with this library's MS-OSM plan at the simulated DM recovers the three-Gaussian profile.
+
+generate_pulsar_signal_new disperses one seed-1 record with one FFT through a causal response: the
+inverse FFT of the chirp on filter_point bins (on the device, or in FP64 on the host for designs below
+2^12 points and at 2^23), its last Nd taps then tap 0. Its FP64 restatement repeats those steps with
+numpy on the same record and chirp; same bar.
 """
 import numpy as np
 import pytest
@@ -53,10 +58,54 @@ def test_simulator_matches_fp64_restatement(case):
         assert step.max() <= 1 and frac < 0.05
 
 
-def test_dedispersing_the_simulation_recovers_the_profile():
+def new_reference(bw, dm, f0, period, repeat):
+    """generate_pulsar_signal_new (simulated_complex_signal.cpp:338-431) in FP64 on the same record."""
+    p = ms.simulate_params_new(bw, dm, f0, period, repeat)
+    rec = ms.simulate_noise_new(bw, dm, f0, period, repeat).astype(np.complex128)
+    h = np.fft.ifft(ms.simulate_chirp(bw, dm, f0, p.filter_point).astype(np.complex128))  # includes 1/filter_point
+    tap = np.zeros(p.fftpoint, dtype=np.complex128)
+    tap[:p.Nd] = h[p.filter_point - p.Nd:]  # the last Nd taps first, then tap 0
+    tap[p.Nd] = h[0]
+    y = np.fft.ifft(np.fft.fft(rec) * np.fft.fft(tap))
+    return p, y[p.start:p.start + p.samples]
+
+
+NEW_CASES = {
+    # name: (bw, dm, f0, period, repeat); the filter design length decides where it is inverted
+    "2^20 record, 2^15 filter on the device": (16e6, 30.0, 1392e6, 1e-3, 32),
+    "2^19 record, 2^10 filter on the host": (16e6, 1.0, 1392e6, 1e-3, 16),
+    "DM 0, one-tap filter": (16e6, 0.0, 1392e6, 1e-3, 8),
+    "2^24 record, 2^23 filter on the host": (128e6, 40.0, 1e9, 0.002048, 4),
+    "2^25 record, 2^24 filter six-step": (128e6, 75.0, 1e9, 0.002048, 8),
+}
+
+
+@pytest.mark.parametrize("case", list(NEW_CASES))
+def test_simulator_new_matches_fp64_restatement(case):
+    bw, dm, f0, period, repeat = NEW_CASES[case]
+    p = ms.simulate_params_new(bw, dm, f0, period, repeat)
+    print(f"case {case}: Np={p.Np} Nd={p.Nd} fftpoint=2^{p.fftpoint.bit_length() - 1} "
+          f"filter_point=2^{p.filter_point.bit_length() - 1}", flush=True)
+    codes, series = ms.simulate_pulsar_new(bw, dm, f0, period, repeat, series=True)
+    assert np.array_equal(codes, codes_of(series))
+    _, ref = new_reference(bw, dm, f0, period, repeat)
+    err = np.linalg.norm(series.astype(np.complex128) - ref) / np.linalg.norm(ref)
+    d = np.abs(codes_of(ref).view(np.uint16).astype(np.int32) - codes.view(np.uint16).astype(np.int32))
+    step = np.minimum(d, 65536 - d)
+    frac = np.mean(codes_of(ref) != codes)
+    print(f"  series rel-L2 {err:.2e}, codes differing {frac:.2e} (max step {step.max()})", flush=True)
+    assert err < 1e-5
+    assert step.max() <= 1 and frac < 0.05
+
+
+@pytest.mark.parametrize("generator", ["block", "new"])
+def test_dedispersing_the_simulation_recovers_the_profile(generator):
     # 16 MHz at 1392 MHz, DM 30 (23231 samples of dispersion, 1.45 periods), 1 ms period
     bw, dm, f0, period, nbins = 16e6, 30.0, 1392e6, 1e-3, 250
-    codes = ms.simulate_pulsar_block(bw, dm, f0, period, 0, 128)
+    if generator == "block":
+        codes = ms.simulate_pulsar_block(bw, dm, f0, period, 0, 128)
+    else:
+        codes = ms.simulate_pulsar_new(bw, dm, f0, period, 128)
     design = ms.Design(bw, f0, [dm])
     batch = 4
     per = batch * design.M

@@ -92,3 +92,43 @@ def test_unsupported_and_bad_arguments_raise():
         ms.simulate_params(0.0, 1.0, 1e9, 1e-3)
     with pytest.raises(ms.MsosmError):
         ms.simulate_params(16e6, 1.0, 1e9, 1e-3, 0, 0)
+
+
+# generate_pulsar_signal_new (simulated_complex_signal.cpp:279-467)
+
+def test_params_new_follow_the_reference_definitions():
+    for bw, dm, f0, period in [BAND, (64e6, 100.0, 1218e6, 5e-3), (32e6, 0.0, 784e6, 2e-3), (128e6, 75.0, 1e9, 0.002048)]:
+        for repeat in (1, 16, 200):
+            p = ms.simulate_params_new(bw, dm, f0, period, repeat)
+            b = ms.simulate_params(bw, dm, f0, period, 0, repeat)
+            assert (p.Np, p.Nd, p.range) == (b.Np, b.Nd, b.range)  # the constructor's, shared
+            guard = p.range + 2
+            assert p.periods == repeat + 2 * guard
+            need = p.periods * p.Np + p.Nd
+            assert p.fftpoint >= need and p.fftpoint < 2 * need and p.fftpoint & (p.fftpoint - 1) == 0
+            assert p.filter_point >= p.Nd + 1 and p.filter_point & (p.filter_point - 1) == 0
+            assert p.filter_point == 1 or p.filter_point < 2 * (p.Nd + 1)
+            assert p.start == guard * p.Np + p.Nd and p.samples == repeat * p.Np
+            assert p.start + p.samples <= p.fftpoint
+
+
+def test_noise_new_is_one_seed1_stream():
+    bw, dm, f0, period = BAND
+    p = ms.simulate_params_new(bw, dm, f0, period, 4)
+    rec = ms.simulate_noise_new(bw, dm, f0, period, 4)
+    guard = p.range + 2
+    # the block generator draws its first guard + dummy - 1 periods from the same seed-1 stream
+    dummy = p.periods - guard + 1
+    b = ms.simulate_params(bw, dm, f0, period, dummy, 1)
+    blk = ms.simulate_noise(bw, dm, f0, period, dummy, 1)
+    off = b.fftpoint - (2 * guard + 128) * b.Np
+    assert np.array_equal(rec[:p.periods * p.Np], blk[off:off + p.periods * p.Np])
+    assert not np.any(rec[p.periods * p.Np:])  # zero padding up to fftpoint
+    assert np.any(rec[(p.periods - 1) * p.Np:p.periods * p.Np])
+
+
+def test_new_bad_arguments_raise():
+    with pytest.raises(ms.MsosmError):
+        ms.simulate_params_new(16e6, 1.0, 1e9, 1e-3, 0)
+    with pytest.raises(ms.MsosmError):
+        ms.simulate_params_new(16e6, -1.0, 1e9, 1e-3, 4)
