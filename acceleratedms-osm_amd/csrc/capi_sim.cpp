@@ -166,6 +166,26 @@ int host_threads() { return std::max(1u, std::min(16u, std::thread::hardware_con
         if (e_ != hipSuccess) return set_error(MSOSM_ERR_HIP, std::string("simulate: ") + hipGetErrorString(e_));     \
     } while (0)
 
+extern "C" int msosm_host_alloc(unsigned long long bytes, void **ptr)
+{
+    if (!ptr) return set_error(MSOSM_ERR_ARG, "host_alloc: null pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return MSOSM_OK;
+    const hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return set_error(MSOSM_ERR_ALLOC, std::string("host_alloc: ") + hipGetErrorString(e));
+    }
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_host_free(void *ptr)
+{
+    if (!ptr) return MSOSM_OK;
+    const hipError_t e = hipHostFree(ptr);
+    return e == hipSuccess ? MSOSM_OK : set_error(MSOSM_ERR_HIP, std::string("host_free: ") + hipGetErrorString(e));
+}
+
 extern "C" int msosm_simulate_params(float bw, float dm, float f0, float period, unsigned long dummy_repeat,
                                      unsigned long valid_repeat, unsigned long long *out)
 {
@@ -373,5 +393,121 @@ extern "C" int msosm_simulate_pulsar_new(float bw, float dm, float f0, float per
     if (out_c64) SIMCHK(hipMemcpyAsync(out_c64, dc64, p.signal_size * sizeof(float2), hipMemcpyDeviceToHost, d.st));
     if (out_u16) SIMCHK(hipMemcpyAsync(out_u16, du16, p.signal_size * sizeof(uint32_t), hipMemcpyDeviceToHost, d.st));
     SIMCHK(hipStreamSynchronize(d.st));
+    return MSOSM_OK;
+}
+
+// ---- generate_pulsar_signal (simulated_complex_signal.cpp:469-706): impulses of 32767 at Np/2 and
+// Np/2 + 2 in each of range consecutive periods, dispersed through the response placed at
+// [order, 2 order) of a 2 order FFT, the first Np samples summed over the periods and repeated
+
+extern "C" int msosm_simulate_params_pulse(float bw, float dm, float f0, float period, unsigned long repeat,
+                                           unsigned long long *out)
+{
+    SimParamsPulse p;
+    if (!out) return set_error(MSOSM_ERR_ARG, "simulate_params_pulse: null output");
+    if (sim_params_pulse(bw, dm, f0, period, repeat, &p) != 0)
+        return set_error(MSOSM_ERR_ARG, "simulate_params_pulse: bw, f0, period > 0, dm > 0 and at least one period");
+    const unsigned long long v[MSOSM_SIM_NPARAMS] = {p.Np, p.Nd, p.range, p.order, p.fftpoint, p.signal_size, 0, 0};
+    std::copy(v, v + MSOSM_SIM_NPARAMS, out);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_simulate_pulsar(float bw, float dm, float f0, float period, unsigned long repeat, int add_noise,
+                                     float snr, int device, uint32_t *out_u16, float *out_c64, unsigned long long len)
+{
+    (void)snr;
+    SimParamsPulse p;
+    if (sim_params_pulse(bw, dm, f0, period, repeat, &p) != 0)
+        return set_error(MSOSM_ERR_ARG, "simulate_pulsar: bw, f0, period > 0, dm > 0 and at least one period");
+    if (add_noise)
+        return set_error(MSOSM_ERR_UNSUPPORTED, "simulate_pulsar: add_noise (the reference's noise branch rewrites "
+                                                "period 0 while other threads copy it, so its record is not defined)");
+    if ((!out_u16 && !out_c64) || len < p.signal_size)
+        return set_error(MSOSM_ERR_ARG, "simulate_pulsar: output buffers of repeat * Np samples needed (" +
+                                            std::to_string(p.signal_size) + ")");
+    const int lg = log2_of(p.fftpoint), lgo = log2_of(p.order);
+    if (!sim_plan_supported(lg))
+        return set_error(MSOSM_ERR_UNSUPPORTED, "simulate_pulsar: FFT length 2^" + std::to_string(lg) +
+                                                    " (supported: 2^12..2^22 and 2^24..2^44)");
+    const bool dev_filter = sim_plan_supported(lgo); // else FP64 on the host (order < 2^12 or 2^23)
+    const long long N = (long long)p.fftpoint, O = (long long)p.order, Np = (long long)p.Np;
+
+    // host: the chirp (:524-546), the impulse train (:596-600), and the response if designed here
+    std::vector<float> H1(2 * (size_t)O);
+    sim_chirp_pulse(bw, dm, f0, p.order, H1.data(), host_threads());
+    std::vector<float2> train(N, make_float2(0.f, 0.f));
+    for (unsigned long i = 0; i < p.range; ++i) {
+        train[i * Np + Np / 2].x = 32767.0f;
+        train[i * Np + Np / 2 + 2].x = 32767.0f;
+    }
+    std::vector<float2> h_host;
+    if (!dev_filter) {
+        // :548-557: IFFT, then each component divided by order
+        std::vector<std::complex<double>> x(O);
+        for (long long i = 0; i < O; ++i) x[i] = {H1[2 * i], H1[2 * i + 1]};
+        const Fft64Handle *fft = design_fft_create((int)O);
+        design_fft_run(fft, x.data(), +1);
+        design_fft_destroy(fft);
+        h_host.resize(O);
+        for (long long i = 0; i < O; ++i)
+            h_host[i] = make_float2((float)x[i].real() / (float)O, (float)x[i].imag() / (float)O);
+    }
+
+    if (device < 0) SIMCHK(hipGetDevice(&device)); // the caller's current device
+    SIMCHK(hipSetDevice(device));
+    SimDev d;
+    SIMCHK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));
+    int cus = 0;
+    SIMCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    float2 *dtrain = nullptr, *dresp = nullptr, *dH = nullptr, *dper = nullptr;
+    uint32_t *du16 = nullptr;
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dtrain), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dresp), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dH), N * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&dper), Np * sizeof(float2)));
+    SIMCHK(d.alloc(reinterpret_cast<void **>(&du16), Np * sizeof(uint32_t)));
+    SimPlan plan;
+    SIMCHK(make_plan(d, lg, cus, &plan));
+    SIMCHK(hipMemcpyAsync(dtrain, train.data(), N * sizeof(float2), hipMemcpyHostToDevice, d.st));
+    // :559-561: the response at [order, 2 order) of fftpoint zeros
+    SIMCHK(hipMemsetAsync(dresp, 0, O * sizeof(float2), d.st));
+    if (dev_filter) {
+        SimPlan oplan;
+        SIMCHK(make_plan(d, lgo, cus, &oplan));
+        float2 *dH1 = nullptr;
+        SIMCHK(d.alloc(reinterpret_cast<void **>(&dH1), O * sizeof(float2)));
+        SIMCHK(hipMemcpyAsync(dH1, H1.data(), O * sizeof(float2), hipMemcpyHostToDevice, d.st));
+        SimFftArgs a{};
+        a.ex_start = 0;
+        a.ex_count = O;
+        a.ex_scale = 1.0f / static_cast<float>(O); // a power of two: the same as dividing by order
+        a.ex_c64 = dresp + O;
+        SIMCHK(run_plan(oplan, dH1, nullptr, +1, SIM_EXTRACT, a, d.st));
+    } else {
+        SIMCHK(hipMemcpyAsync(dresp + O, h_host.data(), O * sizeof(float2), hipMemcpyHostToDevice, d.st));
+    }
+    SimFftArgs a{};
+    SIMCHK(run_plan(plan, dresp, dH, -1, SIM_PLAIN, a, d.st)); // :563-565
+    // :588-611: the train's spectrum times the response's, times 1/fftpoint, back; the first Np
+    // samples. The reference transforms each period's impulses separately and sums the results; the
+    // transforms are linear, so one transform of the whole train gives the same sum.
+    a.chirp = dH;
+    SIMCHK(run_plan(plan, dtrain, dresp, -1, SIM_CHIRP, a, d.st));
+    a.ex_start = 0;
+    a.ex_count = Np;
+    a.ex_scale = 1.0f / static_cast<float>(p.fftpoint);
+    a.ex_c64 = dper;
+    a.ex_u16 = du16;
+    SIMCHK(run_plan(plan, dresp, nullptr, +1, SIM_EXTRACT, a, d.st));
+    std::vector<float2> per(Np);
+    std::vector<uint32_t> codes(Np);
+    SIMCHK(hipMemcpyAsync(per.data(), dper, Np * sizeof(float2), hipMemcpyDeviceToHost, d.st));
+    SIMCHK(hipMemcpyAsync(codes.data(), du16, Np * sizeof(uint32_t), hipMemcpyDeviceToHost, d.st));
+    SIMCHK(hipStreamSynchronize(d.st));
+    // :656-677: the period repeated
+    for (unsigned long r = 0; r < p.repeat; ++r) {
+        if (out_c64) std::copy(per.begin(), per.end(), reinterpret_cast<float2 *>(out_c64) + r * Np);
+        if (out_u16) std::copy(codes.begin(), codes.end(), out_u16 + r * Np);
+    }
     return MSOSM_OK;
 }

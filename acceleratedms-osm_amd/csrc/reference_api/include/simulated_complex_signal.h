@@ -1,5 +1,5 @@
 // Reference-compatible pulsar simulator class (simulated_pulsar/include/simulated_complex_signal.h:8-55)
-// over the C ABI: both generators disperse on the GPU (msosm_simulate_pulsar_block, _new).
+// over the C ABI: the three generators disperse on the GPU (msosm_simulate_pulsar, _block, _new).
 #pragma once
 
 #include <string>
@@ -13,6 +13,9 @@ public:
     void generate_pulsar_signal_block(unsigned long dummy_repeat = 0, unsigned long valid_repeat = 1);
     // simulated_complex_signal.cpp:279-467; fills signal, and signal_u16 in "uint16" mode
     void generate_pulsar_signal_new(unsigned long repeat = 1);
+    // simulated_complex_signal.cpp:469-706; add_noise must be false (as msosm_save_simulated.cpp:115 and
+    // test_msosm_speed.cpp:160 pass), pinned selects page-locked host buffers
+    void generate_pulsar_signal(unsigned long repeat = 1, bool add_noise = false, float SNR = 0, bool pinned = true);
     ~SimulatedComplexSignal();
 
 public:
@@ -28,5 +31,7 @@ public:
     unsigned long range = 0;
 
 private:
+    void release();
+    bool pinned_ = false; // signal and signal_u16 from msosm_host_alloc (page-locked)
     float bw, fs, kdm, dm, f0, w0, period, signal_power = 0.f;
 };

@@ -1,6 +1,6 @@
 // Reference-compatible SimulatedComplexSignal (simulated_pulsar/simulated_complex_signal.cpp:16-41,
-// 74-277, 279-467): the constructor sets the reference's members; generate_pulsar_signal_block and
-// generate_pulsar_signal_new return the same records (noise and chirp as the reference builds them,
+// 74-277, 279-467, 469-706): the constructor sets the reference's members; the three generators return
+// the same records (noise and chirp as the reference builds them,
 // dispersion on the current GPU).
 #include "simulated_complex_signal.h"
 
@@ -40,7 +40,7 @@ void SimulatedComplexSignal::generate_pulsar_signal_block(unsigned long dummy_re
     signal_size = (unsigned long)p[4];
     // whole chunks of 128 periods, as the reference allocates; the record is the first signal_size
     const size_t alloc = (size_t)p[7] * 128 * Np;
-    free(signal_u16);
+    release();
     signal_u16 = (uint16_pair *)malloc(sizeof(uint16_pair) * alloc);
     if (!signal_u16) msosm::check(MSOSM_ERR_ALLOC, "generate_pulsar_signal_block: host buffer");
     memset(signal_u16 + signal_size, 0, sizeof(uint16_pair) * (alloc - signal_size));
@@ -55,9 +55,7 @@ void SimulatedComplexSignal::generate_pulsar_signal_new(unsigned long repeat)
     msosm::check(msosm_simulate_params_new(bw, dm, f0, period, repeat, p), "generate_pulsar_signal_new");
     range = (unsigned long)p[2];
     signal_size = (unsigned long)p[4];
-    free(signal);
-    free(signal_u16);
-    signal_u16 = nullptr;
+    release();
     signal = (fftwf_complex *)malloc(sizeof(fftwf_complex) * signal_size);
     if (data_type == "uint16") signal_u16 = (uint16_pair *)malloc(sizeof(uint16_pair) * signal_size);
     if (!signal || (data_type == "uint16" && !signal_u16))
@@ -67,8 +65,44 @@ void SimulatedComplexSignal::generate_pulsar_signal_new(unsigned long repeat)
                  "generate_pulsar_signal_new");
 }
 
-SimulatedComplexSignal::~SimulatedComplexSignal()
+void SimulatedComplexSignal::generate_pulsar_signal(unsigned long repeat, bool add_noise, float SNR, bool pinned)
 {
-    free(signal);
-    free(signal_u16);
+    unsigned long long p[MSOSM_SIM_NPARAMS];
+    msosm::check(msosm_simulate_params_pulse(bw, dm, f0, period, repeat, p), "generate_pulsar_signal");
+    range = (unsigned long)p[2];
+    signal_size = (unsigned long)p[5];
+    release();
+    const bool u16 = data_type == "uint16";
+    if (pinned) {
+        pinned_ = true;
+        msosm::check(msosm_host_alloc(sizeof(fftwf_complex) * signal_size, reinterpret_cast<void **>(&signal)),
+                     "generate_pulsar_signal");
+        if (u16)
+            msosm::check(msosm_host_alloc(sizeof(uint16_pair) * signal_size, reinterpret_cast<void **>(&signal_u16)),
+                         "generate_pulsar_signal");
+    } else {
+        signal = (fftwf_complex *)malloc(sizeof(fftwf_complex) * signal_size);
+        if (u16) signal_u16 = (uint16_pair *)malloc(sizeof(uint16_pair) * signal_size);
+        if (!signal || (u16 && !signal_u16)) msosm::check(MSOSM_ERR_ALLOC, "generate_pulsar_signal: host buffer");
+    }
+    msosm::check(msosm_simulate_pulsar(bw, dm, f0, period, repeat, add_noise ? 1 : 0, SNR, -1,
+                                       reinterpret_cast<uint32_t *>(signal_u16), reinterpret_cast<float *>(signal),
+                                       signal_size),
+                 "generate_pulsar_signal");
 }
+
+void SimulatedComplexSignal::release()
+{
+    if (pinned_) {
+        (void)msosm_host_free(signal);
+        (void)msosm_host_free(signal_u16);
+    } else {
+        free(signal);
+        free(signal_u16);
+    }
+    signal = nullptr;
+    signal_u16 = nullptr;
+    pinned_ = false;
+}
+
+SimulatedComplexSignal::~SimulatedComplexSignal() { release(); }

@@ -1,6 +1,7 @@
-// Host side of the GPU pulsar simulator: the parameters, the noise periods and the dispersion chirp of
-// SimulatedComplexSignal::generate_pulsar_signal_block (simulated_complex_signal.cpp:16-41, 43-72,
-// 74-195), with the reference's types and expression order (compiled without FMA contraction). The
+// Host side of the GPU pulsar simulator: the parameters, the noise periods and the dispersion chirps of
+// SimulatedComplexSignal's three generators (simulated_complex_signal.cpp:16-41, 43-72, 74-195,
+// 279-345, 469-546), with the reference's types and expression order (compiled without FMA
+// contraction). The
 // random numbers come from the same standard-library engines (std::mt19937, std::normal_distribution),
 // so the noise is the reference's bit for bit.
 #include "sim_host.h"
@@ -117,6 +118,52 @@ void sim_noise(const SimParams &p, float *periods)
     gen.seed(2);
     for (unsigned long k = 0; k < p.valid + p.dummy1; k++, q++)
         test_pulse(periods + 2 * (p.offset + q * p.Np), amp, gen);
+}
+
+int sim_params_pulse(float bw, float dm, float f0, float period, unsigned long repeat, SimParamsPulse *p)
+{
+    SimParams b; // the constructor's Np and Nd
+    if (repeat == 0 || sim_params(bw, dm, f0, period, 0, repeat, &b) != 0) return -1;
+    p->Np = b.Np;
+    p->Nd = b.Nd;
+    p->repeat = repeat;
+    p->signal_size = repeat * b.Np;
+    p->range = (unsigned long)std::ceil((float)b.Nd / (float)b.Np); // :500, in float
+    if (p->range == 0) return -1; // no dispersion: the reference's order would be 2^-inf
+    if (b.Np < 5) return -1; // the second impulse, at Np / 2 + 2, lies inside the period
+    // :506-507: 2^ceil(log2(range Np)), the next power of two
+    p->order = 1;
+    while (p->order < p->range * p->Np) p->order <<= 1;
+    p->fftpoint = 2 * p->order;
+    return 0;
+}
+
+void sim_chirp_pulse(float bw, float dm, float f0, unsigned long order, float *H, int nthreads)
+{
+    const double pi_d = 3.14159265358979323846;
+    const double kdm_d = 4.15e15;
+    const double fs_d = static_cast<double>(bw);
+    const double dm_d = static_cast<double>(dm);
+    const double f0_d = static_cast<double>(f0);
+    const double w0_d = 2.0 * pi_d * f0_d;
+    const double step = 2.0 * pi_d * fs_d / static_cast<double>(order);
+    auto work = [&](unsigned long lo, unsigned long hi) {
+        for (unsigned long i = lo; i < hi; i++) {
+            const double w = step * static_cast<double>(i);
+            const double phase = 4.0 * pi_d * pi_d * kdm_d * dm_d * w * w / ((w + w0_d) * w0_d * w0_d);
+            const unsigned long k = i < order / 2 ? order / 2 + i : i - order / 2;
+            H[2 * k] = static_cast<float>(std::cos(phase));
+            H[2 * k + 1] = static_cast<float>(std::sin(phase));
+        }
+    };
+    const int n = std::max(1, std::min(nthreads, 64));
+    std::vector<std::thread> th;
+    const unsigned long chunk = (order + n - 1) / n;
+    for (int i = 0; i < n; ++i) {
+        const unsigned long lo = std::min<unsigned long>(order, i * chunk), hi = std::min<unsigned long>(order, lo + chunk);
+        if (lo < hi) th.emplace_back(work, lo, hi);
+    }
+    for (auto &t : th) t.join();
 }
 
 void sim_chirp(float bw, float dm, float f0, unsigned long fftpoint, float *H, int nthreads)

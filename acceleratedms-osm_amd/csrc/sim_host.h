@@ -25,8 +25,20 @@ struct SimParamsNew {
     unsigned long start;                // first extracted sample: guard * Np + Nd
 };
 
+// generate_pulsar_signal: one period of a dispersed two-impulse pulse, repeated
+struct SimParamsPulse {
+    unsigned long Np, Nd, range; // range = ceil(Nd / Np) in float, as there
+    unsigned long order;         // impulse-response length: next power of two >= range * Np
+    unsigned long fftpoint;      // 2 order
+    unsigned long repeat, signal_size;
+};
+
 int sim_params(float bw, float dm, float f0, float period, unsigned long dummy_repeat, unsigned long valid_repeat,
                SimParams *p);
+int sim_params_pulse(float bw, float dm, float f0, float period, unsigned long repeat, SimParamsPulse *p);
+// the chirp of generate_pulsar_signal on `order` bins: w = 2 pi fs i / order for i in [0, order), stored
+// at (i + order / 2) mod order (:528-546)
+void sim_chirp_pulse(float bw, float dm, float f0, unsigned long order, float *H, int nthreads);
 void sim_noise(const SimParams &p, float *periods);
 int sim_params_new(float bw, float dm, float f0, float period, unsigned long repeat, SimParamsNew *p);
 // the record: total_periods periods from one seed-1 stream, zero-padded to fftpoint samples

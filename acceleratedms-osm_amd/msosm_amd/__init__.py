@@ -46,6 +46,8 @@ EXPORTS = [
     "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune", "msosm_stream_wait",
     "msosm_simulate_params", "msosm_simulate_noise", "msosm_simulate_chirp", "msosm_simulate_pulsar_block",
     "msosm_simulate_params_new", "msosm_simulate_noise_new", "msosm_simulate_pulsar_new",
+    "msosm_simulate_params_pulse", "msosm_simulate_pulsar",
+    "msosm_host_alloc", "msosm_host_free",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
@@ -169,6 +171,12 @@ def lib():
                                      C.c_int),
         "msosm_simulate_pulsar_new": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_int,
                                        C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
+        "msosm_host_alloc": ([C.c_ulonglong, C.POINTER(C.c_void_p)], C.c_int),
+        "msosm_host_free": ([C.c_void_p], C.c_int),
+        "msosm_simulate_params_pulse": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong,
+                                         C.POINTER(C.c_ulonglong)], C.c_int),
+        "msosm_simulate_pulsar": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_int, C.c_float, C.c_int,
+                                   C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
         "msosm_simulate_pulsar_block": ([C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulong, C.c_ulong, C.c_int,
                                          C.POINTER(C.c_uint32), fp, C.c_ulonglong], C.c_int),
     }
@@ -710,3 +718,33 @@ def simulate_pulsar_new(bw: float, dm: float, f0: float, period: float, repeat: 
                                            codes.ctypes.data_as(C.POINTER(C.c_uint32)),
                                            _fp(c64.view(np.float32)) if series else None, p.samples))
     return (codes, c64) if series else codes
+
+
+class SimParamsPulse(NamedTuple):
+    Np: int          # samples per period
+    Nd: int          # dispersion samples
+    range: int       # ceil(Nd / Np), in float
+    order: int       # response length: next power of two >= range * Np
+    fftpoint: int    # 2 * order
+    samples: int     # samples returned: repeat * Np
+
+
+def simulate_params_pulse(bw: float, dm: float, f0: float, period: float, repeat: int = 1) -> SimParamsPulse:
+    """Sizes of generate_pulsar_signal (simulated_complex_signal.cpp:469-706)."""
+    v = (C.c_ulonglong * 8)()
+    _check(lib().msosm_simulate_params_pulse(bw, dm, f0, period, repeat, v))
+    return SimParamsPulse(*[int(x) for x in v[:6]])
+
+
+def simulate_pulsar(bw: float, dm: float, f0: float, period: float, repeat: int = 1, add_noise: bool = False,
+                    snr: float = 0.0, device: int = 0, series: bool = False):
+    """generate_pulsar_signal's record (a dispersed two-impulse pulse, one period repeated) as uint16
+    pairs (uint32, real in the low half); with series=True also the complex64 series."""
+    p = simulate_params_pulse(bw, dm, f0, period, repeat)
+    codes = np.empty(p.samples, dtype=np.uint32)
+    c64 = np.empty(p.samples, dtype=np.complex64) if series else None
+    _check(lib().msosm_simulate_pulsar(bw, dm, f0, period, repeat, int(add_noise), snr, device,
+                                       codes.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       _fp(c64.view(np.float32)) if series else None, p.samples))
+    return (codes, c64) if series else codes
+

@@ -304,6 +304,11 @@ int msosm_saver_destroy(msosm_saver *saver);
 int msosm_saver_file_name(const char *dir, const char *prefix, const float *dms, int num_dms, int i, char *buf,
                           size_t cap);
 
+/* ---- page-locked host memory, for the reference classes' cudaMallocHost / cudaFreeHost buffers
+ * (e.g. simulated_complex_signal.cpp:484-485, 491). bytes = 0 gives NULL. */
+int msosm_host_alloc(unsigned long long bytes, void **ptr);
+int msosm_host_free(void *ptr);
+
 /* ---- GPU pulsar simulator: SimulatedComplexSignal::generate_pulsar_signal_block
  * (simulated_pulsar/simulated_complex_signal.cpp:74-277) with the dispersion injected on the GPU.
  * bw (= fs), dm, f0, period as the reference's float constructor arguments (:16-29);
@@ -345,6 +350,19 @@ int msosm_simulate_noise_new(float bw, float dm, float f0, float period, unsigne
  * reference's is FP32); the other transforms are FP32 on the device, as the reference's are FP32. */
 int msosm_simulate_pulsar_new(float bw, float dm, float f0, float period, unsigned long repeat, int device,
                               uint32_t *out_u16, float *out_c64, unsigned long long len);
+/* SimulatedComplexSignal::generate_pulsar_signal (simulated_complex_signal.cpp:469-706), as
+ * msosm_save_simulated.cpp:115 and benchmark/test_msosm_speed.cpp:160 call it: impulses of 32767 at
+ * Np/2 and Np/2 + 2 in each of range = ceil(Nd / Np) periods, dispersed through the response (the
+ * inverse FFT of the chirp on order >= range Np bins, placed at [order, 2 order) of a 2 order FFT),
+ * the first Np samples summed over those periods and repeated `repeat` times. out[8] = {Np, Nd, range,
+ * order, 2 order, samples returned, 0, 0}. dm must be > 0 (at 0 the reference's order is undefined);
+ * add_noise must be 0, as both callers pass: the reference's noise branch rewrites period 0 while
+ * other threads copy it, so it returns MSOSM_ERR_UNSUPPORTED (snr is then unused). Deviation: the
+ * response is designed with an FP64 inverse FFT on the host when order is below 2^12 or is 2^23. */
+int msosm_simulate_params_pulse(float bw, float dm, float f0, float period, unsigned long repeat,
+                                unsigned long long *out);
+int msosm_simulate_pulsar(float bw, float dm, float f0, float period, unsigned long repeat, int add_noise, float snr,
+                          int device, uint32_t *out_u16, float *out_c64, unsigned long long len);
 
 #ifdef __cplusplus
 }

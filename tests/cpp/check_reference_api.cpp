@@ -10,6 +10,7 @@
 //   psrdada <file.dada> <dm> <period> <count> <nbins>           msosm_psrdada.cpp:88-153 (dual-pol fold)
 //   simulate <bw> <dm> <f0> <period> <dummy> <valid> <out.u16>  check_msosm.cpp:128-133 (GPU simulator)
 //   simulate_new <bw> <dm> <f0> <period> <repeat> <out.u16> <out.c64>   generate_pulsar_signal_new
+//   simulate_pulse <bw> <dm> <f0> <period> <repeat> <pinned> <out.u16> <out.c64>   msosm_save_simulated.cpp:115
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -50,6 +51,18 @@ int main(int argc, char **argv)
             std::cout << "s" << i << " " << file.pol1_in[i].first << " " << file.pol1_in[i].second << " "
                       << file.pol2_in[i].first << " " << file.pol2_in[i].second << std::endl;
         std::cout << "last " << file.pol1_in[2048].first << " " << file.pol2_in[2048].first << std::endl;
+        return 0;
+    }
+    if (mode == "simulate_pulse") {
+        SimulatedComplexSignal sim(std::stof(argv[2]), std::stof(argv[3]), std::stof(argv[4]), std::stof(argv[5]),
+                                   "uint16");
+        sim.generate_pulsar_signal(std::stoul(argv[6]), false, 0, std::stoi(argv[7]) != 0);
+        std::ofstream of(argv[8], std::ios::binary);
+        of.write(reinterpret_cast<const char *>(sim.signal_u16), sim.signal_size * sizeof(uint16_pair));
+        std::ofstream oc(argv[9], std::ios::binary);
+        oc.write(reinterpret_cast<const char *>(sim.signal), sim.signal_size * sizeof(fftwf_complex));
+        std::cout << "Np " << sim.Np << " Nd " << sim.Nd << " range " << sim.range << " size " << sim.signal_size
+                  << std::endl;
         return 0;
     }
     if (mode == "simulate_new") {

@@ -200,3 +200,16 @@ def test_simulated_complex_signal_new(tmp_path):
     assert np.array_equal(np.fromfile(out, dtype=np.uint32), codes)
     assert np.array_equal(np.fromfile(outc, dtype=np.complex64), series)
 
+
+@pytest.mark.parametrize("pinned", [0, 1])
+def test_simulated_complex_signal_pulse(tmp_path, pinned):
+    # generate_pulsar_signal(repeat, false, 0, pinned), as msosm_save_simulated.cpp:115 calls it
+    bw, dm, f0, period, repeat = 16e6, 30.0, 1392e6, 1e-3, 5
+    out, outc = tmp_path / "sim.u16", tmp_path / "sim.c64"
+    text = _run(["simulate_pulse", bw, dm, f0, period, repeat, pinned, out, outc], tmp_path)
+    p = ms.simulate_params_pulse(bw, dm, f0, period, repeat)
+    assert f"Np {p.Np} Nd {p.Nd} range {p.range} size {p.samples}" in text
+    codes, series = ms.simulate_pulsar(bw, dm, f0, period, repeat, series=True)
+    assert np.array_equal(np.fromfile(out, dtype=np.uint32), codes)
+    assert np.array_equal(np.fromfile(outc, dtype=np.complex64), series)
+

@@ -128,3 +128,57 @@ def test_dedispersing_the_simulation_recovers_the_profile(generator):
     corr = max(np.dot(np.roll(a, s), b) / nbins for s in range(nbins))
     print(f"folded profile vs three-Gaussian power profile: best correlation {corr:.4f}", flush=True)
     assert corr > 0.97
+
+
+def pulse_reference(bw, dm, f0, period):
+    """generate_pulsar_signal (simulated_complex_signal.cpp:469-611) in closed form, FP64: the chirp on
+    `order` bins (w = 2 pi fs i / order stored at (i + order/2) mod order), its inverse FFT placed at
+    [order, 2 order), and the circular convolution of impulses of 32767 at r Np + Np/2 and
+    r Np + Np/2 + 2 (r < range) with it, which is a sum of shifted copies of the response."""
+    p = ms.simulate_params_pulse(bw, dm, f0, period, 1)
+    O, Np = p.order, p.Np
+    i = np.arange(O, dtype=np.float64)
+    step = 2.0 * np.pi * np.float64(np.float32(bw)) / O
+    w = step * i
+    w0 = 2.0 * np.pi * np.float64(np.float32(f0))
+    phase = 4.0 * np.pi * np.pi * 4.15e15 * np.float64(np.float32(dm)) * w * w / ((w + w0) * w0 * w0)
+    H1 = np.empty(O, dtype=np.complex64)
+    H1[np.where(i < O // 2, O // 2 + i, i - O // 2).astype(np.int64)] = (np.cos(phase) + 1j * np.sin(phase)).astype(np.complex64)
+    hpad = np.zeros(2 * O, dtype=np.complex128)
+    hpad[O:] = np.fft.ifft(H1.astype(np.complex128))
+    j = np.arange(Np)
+    out = np.zeros(Np, dtype=np.complex128)
+    for r in range(p.range):
+        for s in (Np // 2, Np // 2 + 2):
+            out += 32767.0 * hpad[(j - r * Np - s) % (2 * O)]
+    return p, out
+
+
+PULSE_CASES = {
+    # name: (bw, dm, f0, period); the response length decides where it is designed
+    "2^16, response 2^15 in one workgroup": (16e6, 30.0, 1392e6, 1e-3),
+    "2^12, response 2^11 on the host": (16e6, 1.0, 1392e6, 1e-4),
+    "2^24 six-step, response 2^23 on the host": (128e6, 40.0, 1e9, 0.002048),
+    "2^25 six-step, response 2^24 six-step": (128e6, 75.0, 1e9, 0.002048),
+}
+
+
+@pytest.mark.parametrize("case", list(PULSE_CASES))
+def test_simulator_pulse_matches_closed_form(case):
+    bw, dm, f0, period = PULSE_CASES[case]
+    repeat = 3
+    p = ms.simulate_params_pulse(bw, dm, f0, period, repeat)
+    print(f"case {case}: Np={p.Np} Nd={p.Nd} range={p.range} order=2^{p.order.bit_length() - 1}", flush=True)
+    codes, series = ms.simulate_pulsar(bw, dm, f0, period, repeat, series=True)
+    assert np.array_equal(codes, codes_of(series))
+    per = series[:p.Np]
+    for r in range(1, repeat):  # one period, repeated
+        assert np.array_equal(series[r * p.Np:(r + 1) * p.Np], per)
+    _, ref = pulse_reference(bw, dm, f0, period)
+    err = np.linalg.norm(per.astype(np.complex128) - ref) / np.linalg.norm(ref)
+    d = np.abs(codes_of(ref).view(np.uint16).astype(np.int32) - codes[:p.Np].view(np.uint16).astype(np.int32))
+    step = np.minimum(d, 65536 - d)
+    frac = np.mean(codes_of(ref) != codes[:p.Np])
+    print(f"  series rel-L2 {err:.2e}, codes differing {frac:.2e} (max step {step.max()})", flush=True)
+    assert err < 1e-5
+    assert step.max() <= 1 and frac < 0.05

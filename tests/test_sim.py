@@ -132,3 +132,31 @@ def test_new_bad_arguments_raise():
         ms.simulate_params_new(16e6, 1.0, 1e9, 1e-3, 0)
     with pytest.raises(ms.MsosmError):
         ms.simulate_params_new(16e6, -1.0, 1e9, 1e-3, 4)
+
+
+# generate_pulsar_signal (simulated_complex_signal.cpp:469-706)
+
+def test_params_pulse_follow_the_reference_definitions():
+    for bw, dm, f0, period in [BAND, (64e6, 100.0, 1218e6, 5e-3), (128e6, 75.0, 1e9, 0.002048), (16e6, 1.0, 1392e6, 1e-4)]:
+        p = ms.simulate_params_pulse(bw, dm, f0, period, 3)
+        b = ms.simulate_params(bw, dm, f0, period, 0, 3)
+        assert (p.Np, p.Nd) == (b.Np, b.Nd)
+        assert p.range == int(np.ceil(np.float32(p.Nd) / np.float32(p.Np)))  # float division there
+        assert p.order == 1 << int(np.ceil(np.log2(float(p.range * p.Np))))
+        assert p.fftpoint == 2 * p.order and p.samples == 3 * p.Np
+
+
+def test_pulse_rejects_zero_dm_and_noise():
+    with pytest.raises(ms.MsosmError):
+        ms.simulate_params_pulse(16e6, 0.0, 1392e6, 1e-3, 1)  # no dispersion: order undefined there
+    # the reference's add_noise branch races on period 0: refused before any device work
+    with pytest.raises(ms.MsosmError) as e:
+        ms.simulate_pulsar(16e6, 30.0, 1392e6, 1e-3, 2, add_noise=True, snr=-10.0)
+    assert e.value.code == ms.ERR_UNSUPPORTED
+
+
+def test_host_alloc_round_trip():
+    # page-locked host memory needs the HIP runtime but no device work; bytes = 0 gives NULL
+    p = ms.C.c_void_p()
+    assert ms.lib().msosm_host_alloc(0, ms.C.byref(p)) == ms.OK and not p.value
+    assert ms.lib().msosm_host_free(None) == ms.OK
