@@ -748,3 +748,23 @@ def simulate_pulsar(bw: float, dm: float, f0: float, period: float, repeat: int 
                                        _fp(c64.view(np.float32)) if series else None, p.samples))
     return (codes, c64) if series else codes
 
+
+# ------------------------------------------------------------------------------------------------
+# Reader of the saved series (python_check/check.py:5-53)
+# ------------------------------------------------------------------------------------------------
+def read_complex_bin(filename, start_index: int = 0, length=None) -> np.ndarray:
+    """The complex samples of a saved <prefix>_DM<dm>.dat file, from sample `start_index`, `length`
+    samples (None: to the end of the file). Each sample is one uint32: the real part in the low 16
+    bits, the imaginary part in the high 16, offset-binary (code - 32768) with code 0 read as 0.
+    Returns complex128, as the reference's reader does."""
+    with open(filename, "rb") as f:
+        f.seek(4 * int(start_index))
+        raw = np.fromfile(f, dtype=np.uint32, count=-1 if length is None else int(length))
+
+    def component(c):
+        v = c.astype(np.float64)
+        v[c != 0] -= 32768.0
+        return v
+
+    return component(raw & np.uint32(0xFFFF)) + 1j * component(raw >> np.uint32(16))
+

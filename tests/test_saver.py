@@ -114,3 +114,26 @@ def test_short_writes_and_efbig_surface_in_join(tmp_path, flags):
         assert len(data) == 2 * 4096 + 100
         assert data[:4096] == bytes([0]) * 4096 and data[4096:8192] == bytes([1]) * 4096
         assert data[8192:] == bytes([2]) * 100
+
+
+def test_read_complex_bin_decodes_saved_words(tmp_path):
+    # python_check/check.py:5-53: low 16 bits real, high 16 imaginary, code - 32768, code 0 read as 0
+    codes = np.array([0x0000, 0x8000, 0x8001, 0x7FFF, 0xFFFF, 0x0001], dtype=np.uint32)
+    values = np.array([0.0, 0.0, 1.0, -1.0, 32767.0, -32767.0])
+    words = (codes[:, None] | (codes[None, ::-1] << np.uint32(16))).reshape(-1)  # every real/imag pairing
+    expect = (values[:, None] + 1j * values[None, ::-1]).reshape(-1)
+    nblock = 1024  # one saver block of uint32 words, written through the pwrite writer
+    block = np.zeros(nblock, dtype=np.uint32)
+    block[:words.size] = words
+    path = str(tmp_path / "x_DM1.dat")
+    sv = ms.Saver([path], block.nbytes, 1, 1, 0)
+    sv.start()
+    sv.push_host(block.view(np.uint8).reshape(1, -1))
+    sv.join()
+    sv.close()
+    got = ms.read_complex_bin(path)
+    assert got.dtype == np.complex128 and got.size == nblock
+    assert np.array_equal(got[:words.size], expect) and not np.any(got[words.size:])
+    # start_index and length count samples (4 bytes each)
+    assert np.array_equal(ms.read_complex_bin(path, 7, 5), expect[7:12])
+    assert np.array_equal(ms.read_complex_bin(path, 30), np.concatenate([expect[30:], np.zeros(nblock - 36)]))
