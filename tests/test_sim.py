@@ -160,3 +160,21 @@ def test_host_alloc_round_trip():
     p = ms.C.c_void_p()
     assert ms.lib().msosm_host_alloc(0, ms.C.byref(p)) == ms.OK and not p.value
     assert ms.lib().msosm_host_free(None) == ms.OK
+
+
+@pytest.mark.parametrize("gen", ["block", "new", "pulse"])
+def test_unsupported_fft_length_is_refused_before_device_work(gen):
+    # 2^23 needs a 2^11 factor, which the in-workgroup engine lacks; the check precedes any HIP call
+    if gen == "block":
+        p = ms.simulate_params(16e6, 30.0, 1392e6, 2e-3, 0, 1)
+        call = lambda: ms.simulate_pulsar_block(16e6, 30.0, 1392e6, 2e-3, 0, 1)
+    elif gen == "new":
+        p = ms.simulate_params_new(16e6, 30.0, 1392e6, 1e-3, 300)
+        call = lambda: ms.simulate_pulsar_new(16e6, 30.0, 1392e6, 1e-3, 300)
+    else:
+        p = ms.simulate_params_pulse(16e6, 30.0, 1392e6, 0.2, 1)
+        call = lambda: ms.simulate_pulsar(16e6, 30.0, 1392e6, 0.2, 1)
+    assert p.fftpoint == 1 << 23
+    with pytest.raises(ms.MsosmError) as e:
+        call()
+    assert e.value.code == ms.ERR_UNSUPPORTED
