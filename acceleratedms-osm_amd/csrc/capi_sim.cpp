@@ -453,6 +453,10 @@ extern "C" int msosm_simulate_pulsar(float bw, float dm, float f0, float period,
             h_host[i] = make_float2((float)x[i].real() / (float)O, (float)x[i].imag() / (float)O);
     }
 
+    // host buffers the device copies into: declared before d, whose destructor synchronises the
+    // stream, so a copy still queued at an early return never lands in freed memory
+    std::vector<float2> per(Np);
+    std::vector<uint32_t> codes(Np);
     if (device < 0) SIMCHK(hipGetDevice(&device)); // the caller's current device
     SIMCHK(hipSetDevice(device));
     SimDev d;
@@ -499,8 +503,6 @@ extern "C" int msosm_simulate_pulsar(float bw, float dm, float f0, float period,
     a.ex_c64 = dper;
     a.ex_u16 = du16;
     SIMCHK(run_plan(plan, dresp, nullptr, +1, SIM_EXTRACT, a, d.st));
-    std::vector<float2> per(Np);
-    std::vector<uint32_t> codes(Np);
     SIMCHK(hipMemcpyAsync(per.data(), dper, Np * sizeof(float2), hipMemcpyDeviceToHost, d.st));
     SIMCHK(hipMemcpyAsync(codes.data(), du16, Np * sizeof(uint32_t), hipMemcpyDeviceToHost, d.st));
     SIMCHK(hipStreamSynchronize(d.st));

@@ -1,7 +1,9 @@
 // pybind11 module msosm_amd._core: a folded-profile engine for DM refinement over the C ABI.
 //
-// A ProfileEngine owns one plan for a batch of DM trials whose segmentation (delay blocks, M,
-// delaycount) is frozen at the starting DMs, plus a primal and a tangent fold. For the current DMs
+// A ProfileEngine owns one plan for a batch of DM trials, plus a primal and a tangent fold. M and
+// the ring depth (delaycount) are fixed at creation for DMs up to dm_max; every set_dms call designs
+// each trial anew at its own DM (delay blocks and H), so the profile of a trial is this library's
+// MS-OSM at exactly that DM (the float DM of the reference interface). For the current DMs
 // it computes the folded profile of a device-resident record (forward), or the profile together
 // with its derivative with respect to each DM (tangent: every block is dedispersed with H and
 // with dH/dDM, and 2 Re(conj(y) dy/dDM) is folded with the primal bins). Nothing is stored between
@@ -35,7 +37,7 @@ class ProfileEngine {
   public:
     ProfileEngine(float bw, float f0, py::array_t<float, py::array::c_style | py::array::forcecast> start_dms,
                   int batch, unsigned long nbins, double period, int in_format, int fftpoint, int phase_mode,
-                  int device)
+                  int device, float dm_max)
         : bw_(bw), f0_(f0), B_(batch), nbins_(nbins), period_(period), in_format_(in_format),
           phase_mode_(phase_mode)
     {
@@ -43,10 +45,13 @@ class ProfileEngine {
         if (D <= 0) throw std::invalid_argument("ProfileEngine: need at least one DM");
         D_ = (int)D;
         start_.assign(start_dms.data(), start_dms.data() + D);
-        // common M: the largest minimum order over the DMs (msosm_prepare.cpp:48-69), or fftpoint/2
-        // when that is not smaller (msosm_gpu_batch.cpp:32-44)
+        dm_max_ = std::max(dm_max, *std::max_element(start_.begin(), start_.end()));
+        // common M: the largest minimum order over the DMs up to dm_max (msosm_prepare.cpp:48-69), or
+        // fftpoint/2 when that is not smaller (msosm_gpu_batch.cpp:32-44)
+        std::vector<float> sizing(start_);
+        sizing.push_back(dm_max_);
         int M = 0;
-        for (float dm : start_) {
+        for (float dm : sizing) {
             int nd = 0, m = 0;
             check(msosm_design_nd(bw, f0, dm, &nd), "design_nd");
             if (nd > 0) check(msosm_design_min_order(nd, &m), "design_min_order");
@@ -56,11 +61,11 @@ class ProfileEngine {
         if (fftpoint > 0 && fftpoint / 2 >= M) M = fftpoint / 2;
         M_ = M;
         delays_.resize((size_t)D_ * 2 * M_);
+        // ring depth for the largest DM allowed (the delaycount grows with the DM at a fixed M)
         int dc = 1;
-        for (int i = 0; i < D_; ++i) {
+        for (float dm : sizing) {
             int dci = 0;
-            check(msosm_design_segmentation(bw, f0, start_[i], M_, delays_.data() + (size_t)i * 2 * M_, &dci),
-                  "design_segmentation");
+            check(msosm_design_segmentation(bw, f0, dm, M_, delays_.data(), &dci), "design_segmentation");
             dc = std::max(dc, dci);
         }
         dc_ = dc;
@@ -88,15 +93,26 @@ class ProfileEngine {
         if (tfold_) msosm_fold_destroy(tfold_);
     }
 
-    // new trial DMs (the segmentation stays that of the starting DMs); with_tangent also designs and
-    // uploads dH/dDM, which the tangent run needs
+    // new trial DMs: each trial's delay blocks (msosm_prepare.cpp:215-284) and H are designed at its
+    // own DM; with_tangent also designs and uploads dH/dDM, which the tangent run needs. The delay
+    // blocks are piecewise constant in DM, so dH/dDM is the derivative of the profile between the DMs
+    // where a bin changes block.
     void set_dms(py::array_t<float, py::array::c_style | py::array::forcecast> dms, bool with_tangent)
     {
         if (dms.size() != D_) throw std::invalid_argument("set_dms: expected " + std::to_string(D_) + " DMs");
         cur_.assign(dms.data(), dms.data() + D_);
+        for (int i = 0; i < D_; ++i)
+            if (!(cur_[i] >= 0.0f) || cur_[i] > dm_max_)
+                throw std::invalid_argument("set_dms: DM " + std::to_string(cur_[i]) + " outside [0, dm_max=" +
+                                            std::to_string(dm_max_) + "] of this engine");
         std::vector<float> H((size_t)D_ * 4 * M_);
         {
             py::gil_scoped_release nogil;
+            for (int i = 0; i < D_; ++i) {
+                int dci = 0;
+                check(msosm_design_segmentation(bw_, f0_, cur_[i], M_, delays_.data() + (size_t)i * 2 * M_, &dci),
+                      "design_segmentation");
+            }
             check(msosm_design_transfer_functions(bw_, f0_, cur_.data(), D_, M_, phase_mode_, H.data(), 0),
                   "design_transfer_functions");
             for (int i = 0; i < D_; ++i)
@@ -176,6 +192,13 @@ class ProfileEngine {
     unsigned long nbins() const { return nbins_; }
     py::array_t<float> dms() const { return py::array_t<float>((py::ssize_t)D_, cur_.data()); }
     py::array_t<float> start_dms() const { return py::array_t<float>((py::ssize_t)D_, start_.data()); }
+    float dm_max() const { return dm_max_; }
+    // the delay block of every bin of trial i at its current DM (int32 [2M])
+    py::array_t<int> delays(int i) const
+    {
+        if (i < 0 || i >= D_) throw std::out_of_range("delays: trial index");
+        return py::array_t<int>((py::ssize_t)2 * M_, delays_.data() + (size_t)i * 2 * M_);
+    }
 
   private:
     float bw_, f0_;
@@ -184,6 +207,7 @@ class ProfileEngine {
     double period_;
     int in_format_, phase_mode_;
     int D_ = 0, M_ = 0, dc_ = 0;
+    float dm_max_ = 0.0f;
     bool tangent_ready_ = false;
     std::vector<float> start_, cur_;
     std::vector<int> delays_;
@@ -198,10 +222,10 @@ PYBIND11_MODULE(_core, m)
     m.doc() = "Folded-profile engine with DM derivatives (MS-OSM dedispersion + fold on the GPU)";
     py::class_<ProfileEngine>(m, "ProfileEngine")
         .def(py::init<float, float, py::array_t<float, py::array::c_style | py::array::forcecast>, int, unsigned long,
-                      double, int, int, int, int>(),
+                      double, int, int, int, int, float>(),
              py::arg("bw"), py::arg("f0"), py::arg("start_dms"), py::arg("batch"), py::arg("nbins"),
              py::arg("period"), py::arg("in_format") = (int)MSOSM_IN_C64, py::arg("fftpoint") = 0,
-             py::arg("phase_mode") = (int)MSOSM_PHASE_FP64, py::arg("device") = 0)
+             py::arg("phase_mode") = (int)MSOSM_PHASE_FP64, py::arg("device") = 0, py::arg("dm_max") = 0.0f)
         .def("set_dms", &ProfileEngine::set_dms, py::arg("dms"), py::arg("with_tangent") = false)
         .def("run", &ProfileEngine::run, py::arg("input"), py::arg("n_samples"), py::arg("tangent") = false)
         .def_property_readonly("M", &ProfileEngine::M)
@@ -210,5 +234,7 @@ PYBIND11_MODULE(_core, m)
         .def_property_readonly("batch", &ProfileEngine::batch)
         .def_property_readonly("nbins", &ProfileEngine::nbins)
         .def_property_readonly("dms", &ProfileEngine::dms)
-        .def_property_readonly("start_dms", &ProfileEngine::start_dms);
+        .def_property_readonly("start_dms", &ProfileEngine::start_dms)
+        .def_property_readonly("dm_max", &ProfileEngine::dm_max)
+        .def("delays", &ProfileEngine::delays, py::arg("i"));
 }

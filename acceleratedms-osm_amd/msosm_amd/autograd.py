@@ -7,8 +7,11 @@ backward recomputes the dedispersed series: each block is dedispersed with H and
 forward pass besides the DMs is kept. The chain rule with the incoming profile gradient gives the
 DM gradient, so any torch loss on the profiles works (``snr_loss`` is the one used for refinement).
 
-The segmentation (M, delay blocks) is frozen at the engine's starting DMs; only the chirp follows
-the trial DMs, so the profile is a smooth function of DM.
+M and the ring depth are fixed when the engine is made (for DMs up to its ``dm_max``); the delay
+blocks and the chirp of every trial are designed at the trial's own DM on each call, so a profile is
+the library's MS-OSM at exactly that DM. DMs pass the reference interface as float32 (one ulp is
+7.6e-6 pc/cm^3 at DM 123). The delay blocks are piecewise constant in DM, so the gradient is that
+of the profile between the DMs where a frequency bin changes block.
 """
 from __future__ import annotations
 
@@ -70,12 +73,15 @@ def profile_noise_var(record: torch.Tensor, n_samples: int, nbins: int) -> float
 
 
 def refine_dm(engine, record: torch.Tensor, dm_start, noise_var: float, n_samples: int | None = None,
-              steps: int = 40, step0: float = 0.02, step_min: float = 1e-6, step_max: float = 0.2):
+              steps: int = 40, step0: float = 0.02, step_min: float = 1e-5, step_max: float = 0.2,
+              bounds: tuple | None = None):
     """Gradient DM refinement of every trial at once with sign-based steps (torch.optim.Rprop).
 
-    Each step runs one forward and one backward (the derivative pass). Returns
-    (refined DMs, final losses, number of steps taken)."""
-    dm = torch.tensor(np.asarray(dm_start, dtype=np.float64), requires_grad=True)
+    Each step runs one forward and one backward (the derivative pass). The smallest step stays above
+    one float32 ulp of the DMs. Trials are kept inside ``bounds`` (default: [0, engine.dm_max]).
+    Returns (refined DMs, final losses, number of steps taken)."""
+    lo, hi = bounds if bounds is not None else (0.0, float(engine.dm_max))
+    dm = torch.tensor(np.clip(np.asarray(dm_start, dtype=np.float64), lo, hi), requires_grad=True)
     opt = torch.optim.Rprop([dm], lr=step0, etas=(0.5, 1.2), step_sizes=(step_min, step_max))
     loss = None
     for it in range(steps):
@@ -83,6 +89,8 @@ def refine_dm(engine, record: torch.Tensor, dm_start, noise_var: float, n_sample
         loss = snr_loss(folded_profile(dm, engine, record, n_samples), noise_var)
         loss.sum().backward()
         opt.step()
+        with torch.no_grad():
+            dm.clamp_(lo, hi)
     with torch.no_grad():
         final = snr_loss(folded_profile(dm.detach(), engine, record, n_samples), noise_var)
     return dm.detach().numpy(), final.numpy(), steps

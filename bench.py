@@ -8,7 +8,10 @@ bins at P = 5.0 ms inside the dedispersion kernel. One step = the whole 2^27-sam
 all 512 trials, input resident in HBM.
 
 Prints one JSON line: DM-trial*samples/s, the dedispersion kernel's roofline position (HIP events
-on the plan's stream) and the CPU oracle timed on a bounded sample of the same workload.
+on the plan's stream), the CPU port timed on a bounded sample of the same workload, the relative
+error of each workload against the FP64 oracle (sampled trials, outside the timed region), a check
+of the timed launch's folds against the sequential launch, and a nested object for the uneven
+1024-trial grid (one launch over all FFT lengths against one launch per trial).
 """
 import argparse
 import json
@@ -50,6 +53,14 @@ REFINE = dict(workload="msosm_dm_refine", samples=1 << 24, fs_hz=16e6, fc_hz=140
 WORKLOAD = dict(workload="msosm_dm_grid_int8", samples=1 << 27, fs_hz=64e6, fc_hz=1250e6, bw_hz=64e6,
                 num_dms=512, dm_lo=0.0, dm_hi=100.0, fft_len=1 << 16, batch=32, nbins=1024, period_s=5.0e-3,
                 input="int8 I/Q, uniform in [-127,127]")
+
+
+def alg_bytes_per_dm_sample(in_bytes: float, out_bytes: float, num_dms: int, fft_len: int, samples: int) -> float:
+    """Algorithmic HBM bytes per DM-trial*sample: S + O + F/D + 16 L/N with S = 16 (the L gathered
+    complex64 bins of a block per M outputs), O the output bytes per sample, F = in_bytes + 16 (input
+    read and its spectrum written once, shared by the D trials) and 16 L/N the chirp read once per trial
+    over the record."""
+    return 16.0 + out_bytes + (in_bytes + 16.0) / num_dms + 16.0 * fft_len / samples
 
 
 def log(*a):
@@ -103,8 +114,10 @@ def main():
     ap.add_argument("--push-blocks", type=int, default=0,
                     help="uneven workload: blocks of the longest length per push (default 4)")
     ap.add_argument("--graph", action="store_true", help="uneven workload: run each one-launch push as a hipGraph")
-    ap.add_argument("--plain-pushes", type=int, default=8,
-                    help="uneven workload: pushes timed in the one-launch-per-DM form")
+    ap.add_argument("--plain-pushes", type=int, default=4,
+                    help="uneven workload: pushes timed in the one-launch-per-DM form (at most a quarter record)")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="default workload: leave out the nested uneven-grid measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -192,45 +205,148 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = D * pushes * per / ((t1 - t0) / args.steps)
 
-    # roofline of the dominant kernel: algorithmic bytes per launch = D*B*M DM-samples x 16 B
-    # (the L = 2M gathered complex64 bins read per M output samples; the fold writes ~nothing)
+    # roofline of the dominant kernel: algorithmic bytes per launch = the per-DM-sample figure
+    # S + O + F/D + 16 L/N (O = 0: the fold writes ~nothing per sample) times the D*B*M DM-samples of
+    # one launch
     inv_avg_s = inv_ms / inv_n / 1e3
-    alg_bytes = D * B * M * 16.0
+    per_dm_sample = alg_bytes_per_dm_sample(2, 0, D, 2 * M, N)
+    alg_bytes = D * B * M * per_dm_sample
     achieved = alg_bytes / inv_avg_s
-    # the role-split schedule launches inv_rs_kernel, the others inv_kernel; each has its own
-    # counter-derived traffic file under profiles/
-    kname = "inv_rs_kernel" if order & SPLIT else "inv_kernel"
+    kname, kdesc = dedispersion_kernel(order, 2 * M)
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "inv_rs_kernel<256>" if order & SPLIT else "inv_kernel<256,256>",
-            "avg_launch_ms": inv_avg_s * 1e3, "launches": inv_n, "alg_bytes_per_launch": alg_bytes}
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_msosm_dm_grid_int8.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+            "frac": achieved / HBM_PEAK, "traffic": None, "kernel": kdesc,
+            "avg_launch_ms": inv_avg_s * 1e3, "launches": inv_n, "alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_per_dm_sample": per_dm_sample}
+    roof.update(read_traffic(kname, "msosm_dm_grid_int8", order_name(order)))
     log(f"fwd kernel avg {fwd_ms / max(fwd_n, 1):.3f} ms x {fwd_n}, inv kernel avg {inv_avg_s * 1e3:.3f} ms x {inv_n}")
 
     prof, sums, cnts = fold.read()
     assert int(cnts.sum()) > 0
+    plan.close()
+    fold.close()
+
+    # outside the timed region: the timed folds of sampled trials against the sequential launch of
+    # the same design over the same pushes (identical counts; sums equal up to FP64 atomic order)
+    check = timed_output_check(ms, design, data, pushes, args.warmup + args.steps, sums, cnts)
+    log(f"timed-output check: {check}")
+    rel_err = {"dm_grid_int8": rel_err_dm_grid(ms, design, data)}
+    log(f"rel err dm grid: {rel_err['dm_grid_int8']}")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(design)
         log(f"cpu baseline: {cpu['value']:.3e} DM-trial*samples/s ({cpu['sample']})")
+    del data
+    torch.cuda.empty_cache()
+
+    nested = None
+    if not args.no_uneven:
+        nested = bench_uneven(args, nested=True)
+        rel_err["uneven_c64"] = nested.pop("rel_err")
 
     out = {
         "metric": "DM-trial*samples/s", "value": value, "unit": "DM-trial*samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": dict(W, schedule=schedule), "roofline": roof, "cpu_baseline": cpu,
+        "rel_err_vs_fp64": rel_err, "timed_output_check": check, "uneven_grid": nested,
     }
     print(json.dumps(out), flush=True)
+
+
+def dedispersion_kernel(order: int, fft_len: int):
+    """(file stem, description) of the kernel a one-length dedispersion launch runs."""
+    if order & SPLIT:
+        n = int(round(fft_len ** 0.5))
+        return "inv_rs_kernel", f"inv_rs_kernel<{n}>"
+    lg = fft_len.bit_length() - 1
+    n1, n2 = 1 << ((lg + 1) // 2), 1 << (lg // 2)
+    return "inv_kernel", f"inv_kernel<{n1},{n2}>"
+
+
+def read_traffic(kname: str, workload: str, schedule: str) -> dict:
+    """L2-to-fabric bytes per launch of a kernel from its committed rocprofv3 counter summary (TCC
+    FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel's readings), if one exists for this schedule."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kname}_{workload}.json")
+    if not os.path.exists(tpath):
+        return {}
+    with open(tpath) as fh:
+        t = json.load(fh)
+    if t.get("schedule") not in (None, schedule):
+        return {"traffic_note": f"counter profile is of schedule {t.get('schedule')}, not {schedule}"}
+    return {"traffic": t.get("l2_fabric_bytes_per_launch"),
+            "traffic_counts": "L2-to-fabric bytes per launch (TCC FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                              "the MALL may serve part before HBM)",
+            "traffic_source": "profiles/" + t.get("source", "")}
+
+
+def timed_output_check(ms, design, data, pushes: int, records: int, sums, cnts, ntrials: int = 8) -> dict:
+    """Fold the same pushes through a plan of sampled trials on the sequential launch (linear order)."""
+    import numpy as np
+
+    W = WORKLOAD
+    pick = np.linspace(1, len(design.dms) - 1, ntrials).astype(int)
+    sub = ms.Design(design.bw, design.f0, design.dms[pick], M=design.M)
+    assert np.array_equal(sub.delays, design.delays[pick])
+    per = W["batch"] * design.M
+    plan = ms.Plan.from_design(sub, W["batch"], in_format=ms.IN_I8, out_flags=ms.OUT_NONE, device=0)
+    fold = ms.Fold(W["period_s"], W["fs_hz"], per, W["nbins"], num_series=ntrials, device=0)
+    plan.attach_fold(fold)
+    plan.set_schedule(0, ms.ORDER_LINEAR)
+    base = data.data_ptr()
+    for _ in range(records):
+        for c in range(pushes):
+            plan.push(base + c * per * 2)
+    _, s2, c2 = fold.read()
     plan.close()
     fold.close()
+    counts_equal = bool(np.array_equal(cnts[pick], c2))
+    rel = float(np.max(np.abs(sums[pick] - s2) / np.maximum(np.abs(s2), 1e-300)))
+    if not counts_equal or rel > 1e-9:
+        raise AssertionError(f"timed launch disagrees with the sequential launch: counts equal {counts_equal}, "
+                             f"max sum rel diff {rel:.3e}")
+    return {"trials": [float(d) for d in design.dms[pick]], "records": records, "reference": "sequential linear launch",
+            "fold_counts_equal": counts_equal, "max_fold_sum_rel_diff": rel}
 
 
-def bench_uneven(args):
-    """Uneven DM grid: the one-launch form over all length classes against one launch per DM."""
+def rel_err_series(got, ref) -> tuple[float, float]:
+    import numpy as np
+
+    l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    mx = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    return l2, mx
+
+
+def rel_err_dm_grid(ms, design, data, ntrials: int = 4, calls: int = 2) -> dict:
+    """Series of sampled trials of the grid (same M, design, int8 input) against the FP64 oracle over the
+    record's first calls pushes, from zero history in both."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    W = WORKLOAD
+    pick = np.linspace(1, len(design.dms) - 1, ntrials).astype(int)
+    sub = ms.Design(design.bw, design.f0, design.dms[pick], M=design.M)
+    per = W["batch"] * design.M
+    n = per * calls
+    q = data[: 2 * n].cpu().numpy()
+    plan = ms.Plan.from_design(sub, W["batch"], in_format=ms.IN_I8, out_flags=ms.OUT_C64, device=0)
+    outs = []
+    for c in range(calls):
+        plan.push(q[2 * c * per: 2 * (c + 1) * per].reshape(-1, 2))
+        outs.append(plan.output_c64())
+    plan.close()
+    got = np.concatenate(outs, axis=1)
+    ref = orc.msosm(orc.int8_to_complex(q), design.M, design.delays[pick], design.H[pick].astype(np.complex128))
+    errs = [rel_err_series(got[j], ref[j]) for j in range(ntrials)]
+    return {"rel_l2_max": max(e[0] for e in errs), "max_abs_rel_max": max(e[1] for e in errs),
+            "trials": [float(d) for d in design.dms[pick]], "samples": n, "bar": 1e-5}
+
+
+def bench_uneven(args, nested: bool = False):
+    """Uneven DM grid: the one-launch form over all length classes against one launch per DM, the latter
+    at the reference speed harness's 32 blocks per call (test_msosm_speed.cpp:119-120) and at the best
+    of a few batch sizes."""
     import numpy as np
     import torch
 
@@ -243,63 +359,106 @@ def bench_uneven(args):
     dms = np.linspace(W["dm_lo"], W["dm_hi"], W["num_dms"]).astype(np.float32)
     t = time.perf_counter()
     design = ms.UnevenDesign(W["bw_hz"], f0, dms)
-    multi = ms.Multi(design, push_blocks=W["push_blocks"], in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
     lengths = {int(2 * m): int((design.M == m).sum()) for m in np.unique(design.M)}
-    log(f"design + plans: {multi.num_classes} length classes {lengths} ({time.perf_counter() - t:.1f} s)")
-    N, D, per = W["samples"], W["num_dms"], multi.push_samples
-    pushes = N // per
-    if args.pushes:
-        pushes = min(pushes, args.pushes)
+    N, D = W["samples"], W["num_dms"]
     dev = torch.device("cuda:0")
     gen = torch.Generator(device=dev).manual_seed(4321)
     data = torch.randn(N, dtype=torch.complex64, device=dev, generator=gen)
     torch.cuda.synchronize()
     base = data.data_ptr()
 
-    def run(n, first=0):
-        for c in range(n):
-            multi.push(base + ((first + c) % (N // per)) * per * 8)
-        multi.synchronize()
-
-    res = {}
-    for name, mode, n in (("one_launch", ms.LAUNCH_PERSISTENT, pushes),
-                          ("per_dm", ms.LAUNCH_PER_DM, min(args.plain_pushes, pushes))):
+    def timed(multi, mode, n_pushes, steps, graph=False):
         multi.set_launch_mode(mode)
-        multi.set_graph(args.graph and mode == ms.LAUNCH_PERSISTENT)
+        multi.set_graph(graph)
+        per = multi.push_samples
+        total = N // per
+
+        def run(n, first=0):
+            for c in range(n):
+                multi.push(base + ((first + c) % total) * per * 8)
+            multi.synchronize()
+
         run(2)
         multi.profile(True)
         t0 = time.perf_counter()
-        steps = args.steps if name == "one_launch" else 1
         for _ in range(steps):
-            run(n)
+            run(n_pushes)
         t1 = time.perf_counter()
         kms, kn = multi.profile_read(1)
         multi.profile(False)
-        res[name] = dict(rate=D * n * per * steps / (t1 - t0), ms_per_push=(t1 - t0) * 1e3 / (n * steps),
-                         kernel_ms=kms, launches=kn, pushes=n * steps)
-        log(f"{name}: {res[name]['rate']:.3e} DM-trial*samples/s, {res[name]['ms_per_push']:.3f} ms/push, "
-            f"{kn} dedispersion launches, {kms:.1f} ms in them")
-    one = res["one_launch"]
-    alg = D * per * 24.0  # 16 B gathered spectrum + 8 B complex64 output per DM-sample
-    # (with --graph the HIP events bracket the whole push graph, forward transforms included)
+        return dict(rate=D * n_pushes * per * steps / (t1 - t0), ms_per_push=(t1 - t0) * 1e3 / (n_pushes * steps),
+                    kernel_ms=kms, launches=kn, pushes=n_pushes * steps, push_blocks=per // int(design.M.max()))
+
+    multi = ms.Multi(design, push_blocks=W["push_blocks"], in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
+    log(f"uneven: {multi.num_classes} length classes {lengths} ({time.perf_counter() - t:.1f} s)")
+    per = multi.push_samples
+    pushes = N // per
+    if args.pushes and not nested:
+        pushes = min(pushes, args.pushes)
+    one = timed(multi, ms.LAUNCH_PERSISTENT, pushes, args.steps, graph=args.graph and not nested)
+    log(f"uneven one launch: {one['rate']:.3e} DM-trial*samples/s, {one['ms_per_push']:.3f} ms/push")
+    multi.close()
+    rel = rel_err_uneven(ms, design, W["push_blocks"], data) if nested else None
+    # the plain form: one launch per DM, at the harness's 32 blocks per call and at other batches
+    per_dm = {}
+    for pb in (4, 32, 64):
+        m2 = ms.Multi(design, push_blocks=pb, in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
+        n_p = max(1, min(args.plain_pushes, (N // m2.push_samples) // 4))
+        per_dm[pb] = timed(m2, ms.LAUNCH_PER_DM, n_p, 1)
+        log(f"uneven per-DM launches at {pb} blocks: {per_dm[pb]['rate']:.3e} DM-trial*samples/s")
+        m2.close()
+    best_pb = max(per_dm, key=lambda k: per_dm[k]["rate"])
+    per_sample = alg_bytes_per_dm_sample(8, 8, D, int(2 * design.M.max()), N)
+    alg = D * per * per_sample
     avg_s = one["kernel_ms"] / one["launches"] / 1e3
     roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": alg / avg_s / HBM_PEAK, "traffic": None, "kernel": "inv_multi_kernel",
-            "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "alg_bytes_per_launch": alg}
-    tpath = os.path.join(ROOT, "profiles", "traffic_inv_multi_kernel_msosm_uneven_c64.json")
-    if os.path.exists(tpath) and not args.graph:
-        with open(tpath) as fh:
-            roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+            "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "alg_bytes_per_launch": alg,
+            "alg_bytes_per_dm_sample": per_sample}
+    if not (args.graph and not nested):
+        roof.update(read_traffic("inv_multi_kernel", "msosm_uneven_c64", "one launch"))
+    del data
+    torch.cuda.empty_cache()
+    plain = {"harness_batch_32": {"value": per_dm[32]["rate"], "pushes": per_dm[32]["pushes"],
+                                  "speedup_of_one_launch": one["rate"] / per_dm[32]["rate"]},
+             "best_batch": {"push_blocks": best_pb, "value": per_dm[best_pb]["rate"],
+                            "pushes": per_dm[best_pb]["pushes"],
+                            "speedup_of_one_launch": one["rate"] / per_dm[best_pb]["rate"]},
+             "tried": {str(k): v["rate"] for k, v in per_dm.items()}}
     out = {
-        "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 2, "ms_per_step": one["ms_per_push"] * pushes, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": dict(W, length_classes=lengths, graph=bool(args.graph)), "roofline": roof, "cpu_baseline": None,
-        "per_dm_launch_form": {"value": res["per_dm"]["rate"], "pushes": res["per_dm"]["pushes"],
-                               "speedup_of_one_launch": one["rate"] / res["per_dm"]["rate"]},
+        "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s",
+        "ms_per_step": one["ms_per_push"] * pushes, "steps": args.steps,
+        "config": dict(W, length_classes=lengths, graph=bool(args.graph and not nested)), "roofline": roof,
+        "per_dm_launch_form": plain,
     }
+    if nested:
+        out["rel_err"] = rel
+        return out
+    out.update({"n_gpus": 1, "warmup": 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic", "cpu_baseline": None})
     print(json.dumps(out), flush=True)
+
+
+def rel_err_uneven(ms, design, push_blocks: int, data) -> dict:
+    """One trial of each length class against the FP64 oracle over the record's first push (zero history)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    multi = ms.Multi(design, push_blocks=push_blocks, in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
+    per = multi.push_samples
+    x = data[:per].cpu().numpy()
+    multi.push(x)
+    multi.synchronize()
+    res = {}
+    for M in np.unique(design.M):
+        d = int(np.nonzero(design.M == M)[0][-1])
+        got = multi.output(d)
+        ref = orc.msosm(x.astype(np.complex128), int(M), design.delays[d][None], design.H[d][None].astype(np.complex128))[0]
+        l2, mx = rel_err_series(got, ref)
+        res[str(int(2 * M))] = {"dm": float(design.dms[d]), "rel_l2": l2, "max_abs_rel": mx}
     multi.close()
+    return {"per_length": res, "rel_l2_max": max(v["rel_l2"] for v in res.values()), "samples": per, "bar": 1e-5}
 
 
 def synth_pulsar(n, fs, f0, dm, period, duty, amp, sigma, seed, device):
@@ -341,11 +500,14 @@ def bench_refine(args):
     noise_var = ag.profile_noise_var(rec, N, W["nbins"])
     rng = np.random.default_rng(20261203)
     starts = W["true_dm"] + rng.uniform(-W["start_halfwidth"], W["start_halfwidth"], W["num_starts"])
-    eng = ag.ProfileEngine(W["bw_hz"], f0, starts.astype(np.float32), W["batch"], W["nbins"], W["period_s"])
+    lo, hi = W["true_dm"] - W["start_halfwidth"], W["true_dm"] + W["start_halfwidth"]
+    # every trial is designed at its own DM on each pass; M and the ring depth cover the whole window
+    eng = ag.ProfileEngine(W["bw_hz"], f0, starts.astype(np.float32), W["batch"], W["nbins"], W["period_s"],
+                           dm_max=float(np.float32(hi)))
     log(f"refine: M={eng.M} delaycount={eng.delaycount} noise_var={noise_var:.3e}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    dm_ref, loss_ref, steps = ag.refine_dm(eng, rec, starts, noise_var, steps=args.refine_steps)
+    dm_ref, loss_ref, steps = ag.refine_dm(eng, rec, starts, noise_var, steps=args.refine_steps, bounds=(lo, hi))
     t_ref = time.perf_counter() - t0
     best = int(np.argmin(loss_ref))
     # the grid: every DM at 1e-4 over +-0.5 around the truth, in engine batches of 1024 trials
@@ -354,13 +516,32 @@ def bench_refine(args):
     gl = []
     for s0 in range(0, grid.size, 1024):
         dms = grid[s0:s0 + 1024].astype(np.float32)
-        ge = ag.ProfileEngine(W["bw_hz"], f0, dms, W["batch"], W["nbins"], W["period_s"])
+        ge = ag.ProfileEngine(W["bw_hz"], f0, dms, W["batch"], W["nbins"], W["period_s"],
+                              dm_max=float(np.float32(hi)))
+        assert ge.M == eng.M
         prof, _, _ = ge.run(rec.data_ptr(), N, False)
         gl.append(ag.snr_loss(torch.from_numpy(prof), noise_var).numpy())
         del ge
     t_grid = time.perf_counter() - t0
     gl = np.concatenate(gl)
     gbest = int(np.argmin(gl))
+    # trend or noise: a line through the means of 20 consecutive grid stretches (the loss is smooth on
+    # 1e-4 scales, so single grid points are not independent), and the pulse smear over the window
+    nblk = 20
+    blk = gl[: (gl.size // nblk) * nblk].reshape(nblk, -1)
+    xm = grid[: blk.size].reshape(nblk, -1).mean(axis=1)
+    ym = blk.mean(axis=1)
+    A = np.vstack([xm - xm.mean(), np.ones(nblk)]).T
+    coef, res, _, _ = np.linalg.lstsq(A, ym, rcond=None)
+    sigma = float(np.sqrt(res[0] / (nblk - 2))) if res.size else 0.0
+    slope_se = sigma / float(np.sqrt(((xm - xm.mean()) ** 2).sum()))
+    kdm = 4.15e15
+    smear = kdm * W["start_halfwidth"] * (f0 ** -2 - (f0 + W["bw_hz"]) ** -2)
+    landscape = {"grid_loss_min": float(gl.min()), "grid_loss_max": float(gl.max()),
+                 "grid_loss_at_true_dm": float(gl[int(np.argmin(np.abs(grid - W["true_dm"])))]),
+                 "trend_slope_per_dm": float(coef[0]), "trend_slope_stderr": slope_se,
+                 "stretch_mean_std": float(ym.std()),
+                 "smear_at_window_edge_us": smear * 1e6, "pulse_fwhm_us": W["duty"] * W["period_s"] * 1e6}
     out = {
         "metric": "DM refinement", "workload": W["workload"], "config": W,
         "refined_best_dm": float(dm_ref[best]), "refined_best_loss": float(loss_ref[best]),
@@ -369,6 +550,7 @@ def bench_refine(args):
         "refine_seconds": t_ref, "refine_steps": steps, "refine_dm_passes": W["num_starts"] * steps * 3,
         "grid_seconds": t_grid, "grid_trials": int(grid.size),
         "refined_dm_spread": [float(np.min(dm_ref)), float(np.max(dm_ref))],
+        "refined_better_than_grid": bool(loss_ref[best] <= gl[gbest]), "landscape": landscape,
     }
     print(json.dumps(out), flush=True)
 

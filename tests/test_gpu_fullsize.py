@@ -95,6 +95,51 @@ def test_dm_grid_int8_full_record():
         assert rel < 1e-5
 
 
+def test_dm_grid_role_split_full_record_matches_sequential():
+    # The launch the benchmark times, at its shape: 512 trials over 0-100 pc/cm^3 at L = 2^16, 32 blocks
+    # per push over the whole 2^27-sample int8 record (I/Q uniform in [-127, 127]), on every CU, in the
+    # role-split form with 16-DM XCD tiles. Against the sequential launch of the same tile order, every
+    # push's complex64 series (all 512 x 2^20 values, compared on the device) is bit-identical, and the
+    # fused 1024-bin folds have identical counts and sums equal up to the order of their FP64 atomics.
+    from tests.util import device_view
+
+    fs, f0, nbins, period, batch, n = 64e6, 1218e6, 1024, 5.0e-3, 32, 1 << 27
+    design = ms.Design(fs, f0, np.linspace(0.0, 100.0, 512).astype(np.float32))
+    assert design.M == 1 << 15 and design.delaycount == 54
+    per = batch * design.M
+    g = torch.Generator(device="cuda:0").manual_seed(1234)
+    q_dev = torch.randint(-127, 128, (2 * n,), dtype=torch.int8, device="cuda:0", generator=g)
+    plans, folds, views = [], [], []
+    for order in (ms.ORDER_XCD_TILE16 | ms.SCHEDULE_SPLIT, ms.ORDER_XCD_TILE16):
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_C64, device=0)
+        fold = ms.Fold(period, fs, per, nbins, num_series=512, device=0)
+        plan.attach_fold(fold)
+        plan.set_schedule(0, order)
+        plans.append(plan)
+        folds.append(fold)
+        views.append(device_view(plan.device_output_c64(), 512 * per * 2, "<u4"))
+    wg, order = plans[0].schedule()
+    assert wg == plans[0].info().workgroups and order & ms.SCHEDULE_SPLIT
+    print(f"role split full record: M={design.M} dc={design.delaycount} workgroups={wg} pushes={n // per}",
+          flush=True)
+    torch.cuda.synchronize()
+    for c in range(n // per):
+        for plan in plans:
+            plan.push(q_dev.data_ptr() + c * 2 * per)
+        for plan in plans:
+            plan.synchronize()
+        if not torch.equal(views[0], views[1]):
+            diff = int((views[0] != views[1]).sum())
+            raise AssertionError(f"push {c}: {diff} series words differ between role split and sequential")
+    _, s0, c0 = folds[0].read()
+    _, s1, c1 = folds[1].read()
+    assert np.array_equal(c0, c1)
+    np.testing.assert_allclose(s0, s1, rtol=1e-12)
+    for p_, f_ in zip(plans, folds):
+        p_.close()
+        f_.close()
+
+
 def test_uneven_grid_full_record():
     # 1024 trials over 0-1000 pc/cm^3, band 784-816 MHz, 2^26 complex64 samples at 32 MS/s, each trial
     # at its own FFT length (2^14..2^18 here), one launch over all lengths per push of 4 x 2^17

@@ -65,7 +65,7 @@ def test_autograd_dm_gradient_matches_oracle_and_differences():
 
     dms = np.array([123.1, 123.456], dtype=np.float32)
     batch, calls, nbins, period = 4, 4, 128, 0.0011
-    eng = ag.ProfileEngine(BW, F0, dms, batch, nbins, period)
+    eng = ag.ProfileEngine(BW, F0, dms, batch, nbins, period, dm_max=float(dms.max()) + 0.01)
     M = eng.M
     n = batch * M * calls
     _, x = make_input(ms.IN_C64, 1 << int(np.ceil(np.log2(n))), BW, F0, 123.456, period, seed=23)
@@ -99,3 +99,62 @@ def test_autograd_dm_gradient_matches_oracle_and_differences():
     for i in range(len(dms)):
         print(f"  DM {dms[i]}: grad {g[i]:.6e} central difference {fd[i]:.6e}", flush=True)
         assert abs(g[i] - fd[i]) <= 2e-2 * abs(g[i]) + 1e-6
+
+
+def test_loss_central_differences_at_configured_step():
+    # Central differences of the loss at the configured step h = 1e-3, on the GPU and in the FP64 oracle
+    # with the same designs at DM +- h (each designed anew at its DM, as the engine does: at DM 123.1 one
+    # bin changes delay block within +-1e-3, in both). The two differences agree to the profiles'
+    # rounding. The derivative pass's gradient is printed beside them: at this step the difference
+    # also carries the loss's variation on DM scales below 1e-3 (the residual chirp re-draws the noise),
+    # so it is a coarse check of the gradient, whose exact checks are the FP64 gradient and h = 2^-14.
+    import torch
+    from msosm_amd import autograd as ag
+
+    dms = np.array([123.1, 123.456], dtype=np.float32)
+    h = 1e-3
+    batch, calls, nbins, period = 4, 4, 128, 0.0011
+    eng = ag.ProfileEngine(BW, F0, dms, batch, nbins, period, dm_max=float(dms.max()) + 0.01)
+    M = eng.M
+    n = batch * M * calls
+    _, x = make_input(ms.IN_C64, 1 << int(np.ceil(np.log2(n))), BW, F0, 123.456, period, seed=23)
+    x = x[:n]
+    rec = torch.from_numpy(x.astype(np.complex64)).cuda()
+    noise_var = ag.profile_noise_var(rec, n, nbins)
+    print(f"case central differences h={h} M={M} dc={eng.delaycount} n={n}", flush=True)
+
+    def gpu_loss(d):
+        with torch.no_grad():
+            return ag.snr_loss(ag.folded_profile(torch.tensor(d, dtype=torch.float64), eng, rec), noise_var).numpy()
+
+    def oracle_loss(d):
+        out = []
+        for v in d.astype(np.float32):
+            dl, _ = orc.segmentation(BW, F0, float(v), M)
+            H = orc.transfer_function(BW, F0, float(v), M)
+            y = orc.msosm(x, M, dl[None], H[None])[0]
+            s = np.zeros(nbins)
+            c = np.zeros(nbins, dtype=np.uint64)
+            t0 = 0.0
+            for k in range(calls):
+                orc.fold(np.abs(y[k * batch * M:(k + 1) * batch * M]) ** 2, nbins, t0, 1.0 / BW, period, s, c)
+                t0 += float(batch * M) * (1.0 / BW)
+            p = s / np.maximum(c, 1)
+            out.append(-np.sum((p - p.mean()) ** 2) / noise_var)
+        return np.array(out)
+
+    dp, dm_ = (dms.astype(np.float64) + h).astype(np.float32), (dms.astype(np.float64) - h).astype(np.float32)
+    fd_gpu = (gpu_loss(dp) - gpu_loss(dm_)) / (dp.astype(np.float64) - dm_.astype(np.float64))
+    fd_orc = (oracle_loss(dp) - oracle_loss(dm_)) / (dp.astype(np.float64) - dm_.astype(np.float64))
+    loss0 = oracle_loss(dms)
+    dm_t = torch.tensor(dms.astype(np.float64), requires_grad=True)
+    ag.snr_loss(ag.folded_profile(dm_t, eng, rec), noise_var).sum().backward()
+    g = dm_t.grad.numpy()
+    for i in range(len(dms)):
+        changed = [int((orc.segmentation(BW, F0, float(v), M)[0] != orc.segmentation(BW, F0, float(dms[i]), M)[0]).sum())
+                   for v in (dp[i], dm_[i])]
+        print(f"  DM {dms[i]}: central difference h=1e-3 GPU {fd_gpu[i]:.6e} oracle {fd_orc[i]:.6e}; "
+              f"gradient {g[i]:.6e}; bins changing block at +h/-h {changed}", flush=True)
+        # each loss within ~4e-6 relative of the oracle's (profile rounding), divided by the 2h step
+        tol = 4e-6 * abs(loss0[i]) / (2 * h) + 1e-3 * abs(fd_orc[i])
+        assert abs(fd_gpu[i] - fd_orc[i]) <= tol

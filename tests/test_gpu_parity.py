@@ -267,3 +267,54 @@ def test_role_split_matches_sequential(band):
         assert np.array_equal(series, ref[0])
         assert np.array_equal(cnts, ref[2])
         np.testing.assert_allclose(sums, ref[1], rtol=1e-12)  # FP64 sums: atomic order only
+
+
+ORDER_NAMES = {ms.ORDER_XCD_DM_MAJOR: "xcd_dm_major", ms.ORDER_LINEAR: "linear",
+               ms.ORDER_XCD_BLOCK_MAJOR: "xcd_block_major", ms.ORDER_XCD_TILE4: "xcd_tile4",
+               ms.ORDER_XCD_TILE8: "xcd_tile8", ms.ORDER_XCD_TILE16: "xcd_tile16", ms.ORDER_XCD_TILE2: "xcd_tile2"}
+
+
+@pytest.mark.parametrize("band", ["L2^14", "L2^16"])
+def test_role_split_every_autotune_candidate(band):
+    # every schedule the autotuner can pick at these lengths (each unit order, on all CUs and on half of
+    # them, split or not) against the sequential linear launch: identical series bits and fold counts.
+    # 40 trials x 8 blocks = 320 units per push, more than one per workgroup, so the tile orders' edge
+    # positions (trials beyond the grid) and the role split's look-ahead across units both occur.
+    import torch
+
+    bw, f0 = (BW, F0) if band == "L2^14" else (64e6, 1218e6)
+    dms = np.linspace(0.0, 50.0 if band == "L2^14" else 100.0, 40).astype(np.float32)
+    batch, calls, nbins, period = 8, 2, 512, 0.0011
+    design = ms.Design(bw, f0, dms)
+    assert 2 * design.M == (1 << 14 if band == "L2^14" else 1 << 16)
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), bw, f0, float(dms[-1]), period, seed=47)
+    dev = torch.from_numpy(np.ascontiguousarray(raw[:n]).view(np.float32)).cuda()
+
+    def run(wg, order):
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
+        fold = ms.Fold(period, bw, per, nbins, num_series=len(dms))
+        plan.attach_fold(fold)
+        plan.set_schedule(wg, order)
+        o = []
+        for c in range(calls):
+            plan.push(dev.data_ptr() + c * per * 8)
+            o.append(plan.output_c64())
+        _, sums, cnts = fold.read()
+        G = plan.info().workgroups
+        plan.close()
+        fold.close()
+        return np.concatenate(o, axis=1), sums, cnts, G
+
+    ref = run(0, ms.ORDER_LINEAR)
+    G = ref[3]
+    half = max(8, (G // 2 // 8) * 8)
+    for wg in (G, half):
+        for order, name in ORDER_NAMES.items():
+            for split in (0, ms.SCHEDULE_SPLIT):
+                print(f"case {band} workgroups={wg} order={name}{'+split' if split else ''}", flush=True)
+                series, sums, cnts, _ = run(wg, order | split)
+                assert np.array_equal(series, ref[0]), f"{name} split={bool(split)} wg={wg}: series differ"
+                assert np.array_equal(cnts, ref[2])
+                np.testing.assert_allclose(sums, ref[1], rtol=1e-12)

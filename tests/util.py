@@ -85,3 +85,18 @@ def write_dada(path, pol1: np.ndarray, pol2: np.ndarray | None, bw_mhz: float, f
     with open(path, "wb") as fh:
         fh.write(hdr.ljust(4096, b"\0"))
         fh.write(frames.tobytes())
+
+
+class _DevArray:
+    """A device buffer of the library (plain pointer) as torch reads it in place."""
+
+    def __init__(self, ptr: int, n: int, typestr: str):
+        self.__cuda_array_interface__ = {"data": (int(ptr), False), "shape": (int(n),), "typestr": typestr,
+                                         "version": 2}
+
+
+def device_view(ptr: int, n: int, typestr: str = "<u4"):
+    """torch tensor over n elements at a device pointer owned by the library (no copy)."""
+    import torch
+
+    return torch.as_tensor(_DevArray(ptr, n, typestr), device="cuda")
