@@ -94,3 +94,44 @@ def refine_dm(engine, record: torch.Tensor, dm_start, noise_var: float, n_sample
     with torch.no_grad():
         final = snr_loss(folded_profile(dm.detach(), engine, record, n_samples), noise_var)
     return dm.detach().numpy(), final.numpy(), steps
+
+
+def grid_losses(record: torch.Tensor, n_samples: int, bw: float, f0: float, dms, noise_var: float, nbins: int,
+                period: float, batch: int, dm_max: float, chunk: int = 1024) -> np.ndarray:
+    """Loss of every DM of a grid (engines of at most chunk trials, each trial designed at its DM)."""
+    dms = np.asarray(dms, dtype=np.float64)
+    out = []
+    for s0 in range(0, dms.size, chunk):
+        eng = ProfileEngine(bw, f0, dms[s0:s0 + chunk].astype(np.float32), batch, nbins, period, dm_max=dm_max)
+        prof, _, _ = eng.run(record.data_ptr(), int(n_samples), False)
+        out.append(snr_loss(torch.from_numpy(prof), noise_var).numpy())
+        del eng
+    return np.concatenate(out)
+
+
+def search_dm(record: torch.Tensor, n_samples: int, bw: float, f0: float, lo: float, hi: float, noise_var: float,
+              nbins: int, period: float, batch: int, coarse_step: float, polish: int = 4, steps: int = 20):
+    """DM search: a coarse grid over [lo, hi], then gradient refinement from its `polish` deepest local
+    minima. The coarse step should be near the DM scale on which the loss decorrelates (the residual
+    chirp at the band edge reaching ~1 rad), so that every basin of the loss holds a coarse point and
+    the refinement only has to descend within one.
+
+    Returns a dict: best DM and loss, the coarse grid's best, the refined start and end points, and
+    the number of DM-record passes spent (coarse trials plus three per refinement step and trial)."""
+    grid = np.arange(lo, hi + 0.5 * coarse_step, coarse_step)
+    dm_max = float(np.float32(hi))
+    gl = grid_losses(record, n_samples, bw, f0, grid, noise_var, nbins, period, batch, dm_max)
+    # local minima of the coarse losses, deepest first
+    left = np.concatenate([[np.inf], gl[:-1]])
+    right = np.concatenate([gl[1:], [np.inf]])
+    minima = np.nonzero((gl <= left) & (gl <= right))[0]
+    minima = minima[np.argsort(gl[minima])][:polish]
+    starts = grid[minima]
+    eng = ProfileEngine(bw, f0, starts.astype(np.float32), batch, nbins, period, dm_max=dm_max)
+    dm, loss, nsteps = refine_dm(eng, record, starts, noise_var, n_samples, steps=steps, step0=coarse_step / 4,
+                                 step_min=1e-5, step_max=coarse_step / 2, bounds=(lo, hi))
+    b = int(np.argmin(loss))
+    return {"best_dm": float(dm[b]), "best_loss": float(loss[b]), "coarse_best_dm": float(grid[int(np.argmin(gl))]),
+            "coarse_best_loss": float(gl.min()), "coarse_trials": int(grid.size), "starts": starts.tolist(),
+            "refined": dm.tolist(), "refined_losses": loss.tolist(),
+            "passes": int(grid.size + len(starts) * (3 * nsteps + 1))}

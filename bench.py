@@ -537,6 +537,18 @@ def bench_refine(args):
     slope_se = sigma / float(np.sqrt(((xm - xm.mean()) ** 2).sum()))
     kdm = 4.15e15
     smear = kdm * W["start_halfwidth"] * (f0 ** -2 - (f0 + W["bw_hz"]) ** -2)
+    # the two-stage search: a grid at 1e-3 (the DM scale on which the loss decorrelates here), then
+    # gradient refinement from its 4 deepest basins
+    t0 = time.perf_counter()
+    srch = ag.search_dm(rec, N, W["bw_hz"], f0, lo, hi, noise_var, W["nbins"], W["period_s"], W["batch"], 1e-3,
+                        polish=4, steps=args.refine_steps)
+    t_srch = time.perf_counter() - t0
+    search = {"best_dm": srch["best_dm"], "best_loss": srch["best_loss"], "seconds": t_srch,
+              "dm_record_passes": srch["passes"], "coarse_step": 1e-3, "coarse_best_dm": srch["coarse_best_dm"],
+              "coarse_best_loss": srch["coarse_best_loss"], "polished_starts": srch["starts"],
+              "abs_dm_difference_to_grid_best": abs(srch["best_dm"] - float(grid[gbest])),
+              "loss_le_grid_best": bool(srch["best_loss"] <= gl[gbest])}
+    log(f"search: {search}")
     landscape = {"grid_loss_min": float(gl.min()), "grid_loss_max": float(gl.max()),
                  "grid_loss_at_true_dm": float(gl[int(np.argmin(np.abs(grid - W["true_dm"])))]),
                  "trend_slope_per_dm": float(coef[0]), "trend_slope_stderr": slope_se,
@@ -551,6 +563,7 @@ def bench_refine(args):
         "grid_seconds": t_grid, "grid_trials": int(grid.size),
         "refined_dm_spread": [float(np.min(dm_ref)), float(np.max(dm_ref))],
         "refined_better_than_grid": bool(loss_ref[best] <= gl[gbest]), "landscape": landscape,
+        "coarse_grid_then_gradient": search,
     }
     print(json.dumps(out), flush=True)
 

@@ -158,3 +158,31 @@ def test_loss_central_differences_at_configured_step():
         # each loss within ~4e-6 relative of the oracle's (profile rounding), divided by the 2h step
         tol = 4e-6 * abs(loss0[i]) / (2 * h) + 1e-3 * abs(fd_orc[i])
         assert abs(fd_gpu[i] - fd_orc[i]) <= tol
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_dm_search_reaches_fine_grid_optimum(seed):
+    # DM refinement at reduced size: 2^22 samples of the refinement workload's pulsar (band 1392-1408 MHz,
+    # DM 123.456, P = 33 ms, 3 % FWHM, amplitude 0.5, AWGN sigma 1), 512-bin folds, window +-0.05. The
+    # search (a 1e-3 grid, then gradient refinement from its 4 deepest basins) ends at a loss no worse
+    # than the best of the 1e-4 grid over the same window, within 1e-4 of that grid's best DM, with a
+    # third of the grid's DM-record passes.
+    import torch
+    from msosm_amd import autograd as ag
+
+    n, nbins, period, true, hw = 1 << 22, 512, 33e-3, 123.456, 0.05
+    x = orc.pulsar(n, BW, F0, true, period, duty=0.03, amp=0.5, sigma=1.0, seed=seed).astype(np.complex64)
+    rec = torch.from_numpy(x).cuda()
+    nv = ag.profile_noise_var(rec, n, nbins)
+    lo, hi = true - hw, true + hw
+    print(f"case dm search seed={seed} n={n} window=[{lo}, {hi}]", flush=True)
+    r = ag.search_dm(rec, n, BW, F0, lo, hi, nv, nbins, period, 32, 1e-3, polish=4, steps=20)
+    grid = np.arange(lo, hi + 0.5e-4, 1e-4)
+    gl = ag.grid_losses(rec, n, BW, F0, grid, nv, nbins, period, 32, float(np.float32(hi)))
+    gb = int(np.argmin(gl))
+    print(f"  search: DM {r['best_dm']:.6f} loss {r['best_loss']:.4f} in {r['passes']} passes; "
+          f"1e-4 grid: DM {grid[gb]:.6f} loss {gl[gb]:.4f} in {grid.size} passes", flush=True)
+    assert all(lo <= d <= hi for d in r["refined"])
+    assert r["best_loss"] <= gl[gb]
+    assert abs(r["best_dm"] - grid[gb]) < 1e-4
+    assert r["passes"] * 2 < grid.size
