@@ -282,6 +282,12 @@ struct msosm_plan {
     long long prof_n[2] = {0, 0};
     std::vector<char> dm_set;
     int launch_mode = MSOSM_LAUNCH_PERSISTENT;
+    // accumulate launch (lengths 2^14..2^16, delays < 256): ring, H and dH also in its layout
+    // (perm_pos; the same arrays when R = 1) and the delays as bytes in thread order
+    bool accum_ok = false;
+    int perm_rl = 0, perm_sl = 0;
+    float2 *ring_p = nullptr, *H_p = nullptr, *Hdot_p = nullptr;
+    uint8_t *delay_acc = nullptr;
     int sched_wg = 0, order = MSOSM_ORDER_XCD_DM_MAJOR; // tile schedule (0 workgroups = G)
     // DM-derivative pass: dH/dDM per DM and the per-push primal series it is paired with
     float2 *Hdot = nullptr, *tan_y = nullptr;
@@ -302,7 +308,8 @@ static void plan_free(msosm_plan *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
-                    p->bins, p->out_u16, p->out_f16, p->out_scale, p->Hdot, p->tan_y};
+                    p->bins, p->out_u16, p->out_f16, p->out_scale, p->Hdot, p->tan_y, p->ring_p, p->H_p, p->Hdot_p,
+                    p->delay_acc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : p->in_ev)
@@ -370,8 +377,17 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (tiles_lds + INV_EXTRA_LDS > max_lds)
         return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(tiles_lds) +
                                                " B of LDS per workgroup, device allows " + std::to_string(max_lds));
-    p->max_fold_bins = std::min(2048, (max_lds - tiles_lds - INV_EXTRA_LDS) / 12);
+    // the accumulate form stores the delays as bytes: blocks 0..255
+    p->accum_ok = accum_supported(log2L) && p->dc <= 256;
+    const int acc_lds = p->accum_ok ? accum_lds_bytes(log2L) : 0;
+    if (p->accum_ok) {
+        p->perm_rl = log2L - 14;
+        p->perm_sl = 14;
+    }
+    p->max_fold_bins = std::min(2048, (max_lds - std::max(tiles_lds, acc_lds) - INV_EXTRA_LDS) / 12);
     e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12 + INV_EXTRA_LDS);
+    if (e == hipSuccess && acc_lds)
+        e = configure_accum_lds(log2L, acc_lds + p->max_fold_bins * 12 + INV_EXTRA_LDS);
     if (e != hipSuccess) return fail(MSOSM_ERR_HIP, std::string("plan_create: LDS attribute: ") + hipGetErrorString(e));
     p->G = std::max(8, (cus / 8) * 8);
 
@@ -422,6 +438,14 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (e == hipSuccess) e = dalloc(p, &p->H, D * L * sizeof(float2));
     if (e == hipSuccess) e = dalloc(p, &p->delay, D * L * sizeof(uint16_t));
     if (e == hipSuccess) e = dalloc(p, &p->scratch, (size_t)p->G * L * sizeof(float2));
+    if (e == hipSuccess && p->accum_ok) {
+        e = dalloc(p, &p->delay_acc, D * L);
+        if (e == hipSuccess) e = hipMemset(p->delay_acc, 0, D * L);
+        if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->ring_p, (size_t)p->FBS * L * sizeof(float2));
+        if (e == hipSuccess && p->perm_rl) e = hipMemset(p->ring_p, 0, (size_t)p->FBS * L * sizeof(float2));
+        if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->H_p, D * L * sizeof(float2));
+        if (e == hipSuccess && p->perm_rl) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
+    }
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_U16)) e = dalloc(p, &p->out_u16, D * B * M * sizeof(uint32_t));
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_F16)) e = dalloc(p, &p->out_f16, D * B * M * sizeof(uint32_t));
@@ -443,13 +467,20 @@ extern "C" int msosm_plan_destroy(msosm_plan *p)
     return MSOSM_OK;
 }
 
-// copy L complex values to the device, multiplied by 1/L (exact for normal values: a power of two)
-static hipError_t upload_scaled(float2 *dst, const float *src, int L)
+// copy L complex values to the device multiplied by 1/L (exact for normal values: a power of two), in
+// natural order and, for a plan with a separate accumulate layout, also in that layout (bin k at
+// perm_pos(k))
+static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src)
 {
-    std::vector<float> tmp(2 * (size_t)L);
+    const int L = p->L;
+    std::vector<float2> tmp((size_t)L);
     const float s = std::ldexp(1.0f, -ilog2(L));
-    for (size_t j = 0; j < tmp.size(); ++j) tmp[j] = src[j] * s;
-    return hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice);
+    for (int k = 0; k < L; ++k) tmp[k] = make_float2(src[2 * k] * s, src[2 * k + 1] * s);
+    hipError_t e = hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess || !dst_p) return e;
+    std::vector<float2> perm((size_t)L);
+    for (int k = 0; k < L; ++k) perm[perm_pos((uint32_t)k, p->perm_rl, p->perm_sl)] = tmp[k];
+    return hipMemcpy(dst_p, perm.data(), perm.size() * sizeof(float2), hipMemcpyHostToDevice);
 }
 
 extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, const float *H)
@@ -468,7 +499,18 @@ extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, 
     HIPCHK(hipStreamSynchronize(p->stream));
     HIPCHK(hipMemcpy(p->delay + (size_t)i * p->L, dl.data(), p->L * sizeof(uint16_t), hipMemcpyHostToDevice));
     // the device copy carries the inverse transform's 1/L (exact: a power of two)
-    HIPCHK(upload_scaled(p->H + (size_t)i * p->L, H, p->L));
+    HIPCHK(upload_scaled(p, p->H + (size_t)i * p->L, p->H_p ? p->H_p + (size_t)i * p->L : nullptr, H));
+    if (p->accum_ok) {
+        // byte delays in thread order: position r S + p + 1024 t (perm_pos) goes to byte r S + 16 p + t
+        std::vector<uint8_t> db((size_t)p->L);
+        const uint32_t S = 1u << 14;
+        for (int k = 0; k < p->L; ++k) {
+            const uint32_t pos = perm_pos((uint32_t)k, p->perm_rl, p->perm_sl);
+            const uint32_t r = pos / S, q = pos % S;
+            db[(size_t)r * S + 16 * (q & 1023u) + (q >> 10)] = (uint8_t)delay_points[k];
+        }
+        HIPCHK(hipMemcpy(p->delay_acc + (size_t)i * p->L, db.data(), db.size(), hipMemcpyHostToDevice));
+    }
     p->dm_set[i] = 1;
     return MSOSM_OK;
 }
@@ -484,11 +526,13 @@ extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
         hipError_t e = dalloc(p, &p->Hdot, hb);
         if (e == hipSuccess) e = dalloc(p, &p->tan_y, yb);
         if (e == hipSuccess) e = hipMemset(p->Hdot, 0, hb);
+        if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->Hdot_p, hb);
+        if (e == hipSuccess && p->perm_rl) e = hipMemset(p->Hdot_p, 0, hb);
         if (e != hipSuccess)
             return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
         p->tan_set.assign(p->D, 0);
     }
-    HIPCHK(upload_scaled(p->Hdot + (size_t)i * p->L, dH, p->L));
+    HIPCHK(upload_scaled(p, p->Hdot + (size_t)i * p->L, p->Hdot_p ? p->Hdot_p + (size_t)i * p->L : nullptr, dH));
     p->tan_set[i] = 1;
     return MSOSM_OK;
 }
@@ -545,6 +589,9 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     fa.tw_lo = tw + p->tw_lo;
     fa.tw_hi = tw + p->tw_hi;
     fa.tw_s = p->tw_s;
+    fa.ring_p = p->ring_p; // a second copy only where the accumulate layout differs
+    fa.perm_rl = p->perm_rl;
+    fa.perm_sl = p->perm_sl;
     a.log2L = p->log2L;
     a.fwd_grid = std::min(p->B, p->G);
     a.hist = HistArgs{p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M};
@@ -567,6 +614,9 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     ia.tw_hi = fa.tw_hi;
     ia.tw_s = p->tw_s;
     ia.order = p->order;
+    ia.ring_p = p->ring_p ? p->ring_p : p->ring;
+    ia.H_p = p->H_p ? p->H_p : p->H;
+    ia.delay_acc = p->delay_acc;
     if (p->fold) {
         msosm_fold *f = p->fold;
         a.has_bins = true;
@@ -680,6 +730,8 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
                 one.D = 1;
                 one.H = x.H + (size_t)d * p->L;
                 one.delay = x.delay + (size_t)d * p->L;
+                one.H_p = x.H_p + (size_t)d * p->L;
+                if (x.delay_acc) one.delay_acc = x.delay_acc + (size_t)d * p->L;
                 if (x.out_c64) one.out_c64 = x.out_c64 + d * ser;
                 if (x.out_u16) one.out_u16 = x.out_u16 + d * ser;
                 if (x.out_f16) one.out_f16 = x.out_f16 + d * ser;
@@ -709,6 +761,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
         if (run_inv(prim) != MSOSM_OK) return MSOSM_ERR_HIP;
         InvArgs tan = prim;
         tan.H = p->Hdot;
+        tan.H_p = p->Hdot_p ? p->Hdot_p : p->Hdot;
         tan.out_c64 = nullptr;
         tan.tangent_ref = p->tan_y;
         tan.bins = p->bins;
@@ -799,6 +852,7 @@ extern "C" int msosm_plan_reset(msosm_plan *p)
     if (!p) return set_error(MSOSM_ERR_ARG, "plan_reset: null plan");
     HIPCHK(hipSetDevice(p->device));
     HIPCHK(launch_fill(p->ring, (long long)p->FBS * p->L * sizeof(float2), 0u, p->stream));
+    if (p->ring_p) HIPCHK(launch_fill(p->ring_p, (long long)p->FBS * p->L * sizeof(float2), 0u, p->stream));
     HIPCHK(launch_fill(p->hist, (long long)p->M * sizeof(float2), 0u, p->stream));
     p->g0 = 0;
     return MSOSM_OK;
@@ -847,7 +901,8 @@ extern "C" int msosm_plan_set_schedule(msosm_plan *p, int workgroups, int order)
     if (!p) return set_error(MSOSM_ERR_ARG, "plan_set_schedule: null plan");
     if (workgroups < 0 || workgroups > p->G)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: workgroups must be in [0, " + std::to_string(p->G) + "]");
-    if ((order & ~(MSOSM_ORDER_MASK | MSOSM_SCHEDULE_SPLIT)) || (order & MSOSM_ORDER_MASK) > MSOSM_ORDER_XCD_TILE2)
+    if ((order & ~(MSOSM_ORDER_MASK | MSOSM_SCHEDULE_SPLIT | MSOSM_SCHEDULE_ACCUM)) ||
+        (order & MSOSM_ORDER_MASK) > MSOSM_ORDER_XCD_TILE2)
         return set_error(MSOSM_ERR_ARG, "plan_set_schedule: unknown order");
     p->sched_wg = workgroups;
     p->order = order;
@@ -877,11 +932,14 @@ extern "C" int msosm_plan_autotune(msosm_plan *p, const void *input, int pushes,
     const int wgs[2] = {p->G, half};
     double best = 1e300;
     int bw = p->G, bo = MSOSM_ORDER_XCD_DM_MAJOR;
-    // role-split candidates where the launch has that form (FFT length, plain epilogue)
-    const int nsplit = split_supported(p->log2L) && !(p->out_flags & MSOSM_OUT_F16) ? 2 : 1;
+    // launch forms: sequential, role split (FFT length, plain epilogue) and accumulate (FFT length)
+    std::vector<int> forms{0};
+    if (split_supported(p->log2L) && !(p->out_flags & MSOSM_OUT_F16)) forms.push_back(MSOSM_SCHEDULE_SPLIT);
+    if (p->accum_ok) forms.push_back(MSOSM_SCHEDULE_ACCUM);
+    const int norders = MSOSM_ORDER_XCD_TILE2 + 1;
     for (int wi = 0; wi < (half < p->G ? 2 : 1); ++wi)
-        for (int c = 0; c < (MSOSM_ORDER_XCD_TILE2 + 1) * nsplit; ++c) {
-            const int o = (c % (MSOSM_ORDER_XCD_TILE2 + 1)) | (c > MSOSM_ORDER_XCD_TILE2 ? MSOSM_SCHEDULE_SPLIT : 0);
+        for (int c = 0; c < norders * (int)forms.size(); ++c) {
+            const int o = (c % norders) | forms[c / norders];
             p->sched_wg = wgs[wi];
             p->order = o;
             int rc = msosm_plan_reset(p);

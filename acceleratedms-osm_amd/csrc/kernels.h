@@ -19,7 +19,17 @@ struct FwdArgs {
     float2 *scratch;       // [grid][L]
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
+    float2 *ring_p;        // [FBS][L] the same spectra in the accumulate layout (perm_pos), or null
+    int perm_rl, perm_sl;  //   its bin k at ((k mod 2^rl) << sl) | (k >> rl)
 };
+
+// Position of spectrum bin k in the accumulate layout of a plan's ring and chirp tables:
+// P(k) = (k mod R) S + k div R with R = 2^rl, S = 2^sl (the accumulate launch's decimation in time;
+// rl = 0 is the natural order).
+__host__ __device__ __forceinline__ uint32_t perm_pos(uint32_t k, int rl, int sl)
+{
+    return ((k & ((1u << rl) - 1u)) << sl) | (k >> rl);
+}
 
 struct InvArgs {
     const float2 *ring;
@@ -41,6 +51,12 @@ struct InvArgs {
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
     int order;                // unit order (MSOSM_ORDER_*): which units run together
+    // accumulate launch (lengths 2^14..2^16): ring and H in the layout perm_pos with R = L / 2^14, and
+    // the delays as bytes in thread order: the 16 delays a thread gathers for transform r are the 16
+    // bytes at r 2^14 + 16 p (p = its column index in [0, 1024)), the t-th one being bin position
+    // r 2^14 + p + 1024 t
+    const float2 *ring_p, *H_p;
+    const uint8_t *delay_acc;
 };
 
 // One persistent launch over several FFT lengths ("length classes"): class c owns the units
@@ -102,6 +118,11 @@ hipError_t launch_inv_multi(const MultiInvArgs &a, int grid, hipStream_t st);
 void split_length(int log2L, int *n1, int *n2);
 bool length_supported(int log2L);
 bool split_supported(int log2L); // a role-split dedispersion kernel exists for this length
+// accumulate form (kernels_accum.hip): lengths 2^14..2^16, spectrum layout R = L / 2^14
+bool accum_supported(int log2L);
+int accum_lds_bytes(int log2L);
+hipError_t configure_accum_lds(int log2L, int bytes);
+hipError_t launch_accum(int log2L, const InvArgs &a, int grid, hipStream_t st);
 int tiles_lds_bytes(int log2L); // tiles and LDS twiddle tables of the forward and dedispersion kernels
 // LDS the inverse kernel needs beyond its tiles and fold histogram (FP16 scale reduction)
 constexpr int INV_EXTRA_LDS = 16;

@@ -31,6 +31,7 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
         __syncthreads();
         const long long slot = (a.g0 + b) % a.FBS;
         float2 *dst = a.ring + (size_t)slot * L;
+        float2 *dstp = a.ring_p ? a.ring_p + (size_t)slot * L : nullptr;
         auto prep = [](int, bool) { return 0; };
         auto emit = [&](const float2 *lds, int tile, bool active, int) {
             if (!active) return;
@@ -40,7 +41,10 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
             for (int i = 0; i < 16; ++i) {
                 const int f = lt + 256 * i;
                 const int n1 = f / NC, row = f % NC;
-                st32(dst, (uint32_t)(tile * NC + row + N2 * n1), lds[lds_pad(lt) + 272 * i]); // pad(lt + 256 i)
+                const uint32_t k = (uint32_t)(tile * NC + row + N2 * n1);
+                const float2 y = lds[lds_pad(lt) + 272 * i]; // pad(lt + 256 i)
+                st32(dst, k, y);
+                if (dstp) st32(dstp, perm_pos(k, a.perm_rl, a.perm_sl), y);
             }
         };
         pass_b<N1, N2, -1>(scr, prep, emit, lds_all, tw);
@@ -291,6 +295,8 @@ hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st)
 
 hipError_t launch_inv(int log2L, const InvArgs &a, int grid, hipStream_t st)
 {
+    if ((a.order & MSOSM_SCHEDULE_ACCUM) && accum_supported(log2L) && a.delay_acc)
+        return launch_accum(log2L, a, grid, st);
     switch (log2L) {
 #define X(lg, a_, b_)                                                                                                  \
     case lg: return launch_inv_t<a_, b_>(a, grid, st);

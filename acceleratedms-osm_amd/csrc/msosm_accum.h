@@ -1,0 +1,301 @@
+// Dedispersion units without a scratch slot, for FFT lengths L = R * S with S = 2^14 and R = 1, 2, 4.
+//
+// A unit (DM d, block b) computes y = IDFT_L(Y), Y[k] = X_{b - delay(k)}[k] H[k], and keeps y[M + m],
+// m in [0, M), M = L / 2. Decimation in time by R splits the inverse transform into R transforms of
+// S points over the bins k = R k' + r:
+//   C_r[j] = sum_{k'} Y[R k' + r] e^{+2 pi i k' j / S},   y[n] = sum_r e^{+2 pi i r n / L} C_r[n mod S].
+// Each C_r is a four-step 128 x 128 transform that lives in the workgroup's LDS (128 KiB), and the kept
+// outputs are accumulated in registers across r (M / 1024 complex values per thread):
+//   R = 1: y[M + m] = C_0[S/2 + m];
+//   R = 2: y[M + j] = C_0[j] - W^j C_1[j];
+//   R = 4: y[M + j] = sum_r (-1)^r W^{rj} C_r[j],  y[M + S + j] = sum_r (-i)^r W^{rj} C_r[j]
+// with W = e^{+2 pi i / L} and j in [0, S). The four-step intermediate never leaves the CU: the launch
+// reads the gathered spectra (ring, chirp, delays) and writes only its outputs.
+//
+// The launch reads its own copies of the ring and chirp, with bin k at position P(k) = (k mod R) S + k div R
+// (perm_pos), so that the gather of C_r reads contiguous runs, and the delays as bytes in thread order
+// (one 16-byte load per thread and transform).
+#pragma once
+#include "msosm_device.h"
+
+namespace msosm {
+
+constexpr int ACC_S = 1 << 14, ACC_N = 128; // sub-transform length and its four-step split
+constexpr int ACC_XS = ACC_N + 8;           // row stride of the LDS transpose (conflict-free both ways)
+constexpr int ACC_AREA = 4 * TileWords<ACC_N>::V; // float2 words: tile exchanges / transpose / fold buffer
+static_assert(ACC_AREA >= ACC_N * ACC_XS, "the transpose fits the tile area");
+
+// Twiddle tables of an accumulate launch in LDS: the 128-point sub-transform table and the split
+// W_L^m = hi[m >> s] lo[m & (2^s - 1)].
+struct AccTw {
+    const float2 *t128, *lo, *hi;
+    int s;
+};
+
+template <int R>
+struct AccShape {
+    static constexpr int L = R * ACC_S, M = L / 2, MPT = M / WG; // kept outputs per thread
+    static constexpr int FOLD_X = WG + 1;                        // fold buffer row stride (floats)
+    static constexpr int TW_WORDS = 128 + 256 + 256;             // t128, lo (<= 256), hi (<= 256)
+    static constexpr int LDS_BYTES = ACC_AREA * 8 + TW_WORDS * 8;
+    static_assert(MPT * FOLD_X <= ACC_AREA * 2, "fold buffer fits");
+};
+
+// the tables of the plan (one-length layout: n1, n2, lo, hi with s = (log2 L + 1) / 2) into LDS
+template <int R>
+__device__ __forceinline__ AccTw acc_load_twiddles(float2 *lds_all, const InvArgs &a)
+{
+    using A = AccShape<R>;
+    float2 *t = lds_all + ACC_AREA;
+    float2 *lo = t + 128, *hi = lo + 256;
+    // a 128-point table: the plan's N2 table has 128 entries for L <= 2^15, its N1 table 256 at 2^16
+    const float2 *src128 = R == 4 ? a.tw_n1 : a.tw_n2;
+    constexpr int st = R == 4 ? 2 : 1;
+    const int s = a.tw_s, nlo = 1 << s, nhi = A::L >> s;
+    for (int i = threadIdx.x; i < 128; i += blockDim.x) t[i] = src128[i * st];
+    for (int i = threadIdx.x; i < nlo; i += blockDim.x) lo[i] = a.tw_lo[i];
+    for (int i = threadIdx.x; i < nhi; i += blockDim.x) hi[i] = a.tw_hi[i];
+    __syncthreads();
+    return AccTw{t, lo, hi, s};
+}
+
+__device__ __forceinline__ float2 acc_wl(const AccTw &tw, int m)
+{
+    return cmul(tw.hi[m >> tw.s], tw.lo[m & ((1 << tw.s) - 1)]);
+}
+
+// The 16 byte delays of this thread for transform r of DM d: the bytes at r S + 16 p (p = k1 + 128 jtA),
+// the t-th one for bin position r S + p + 1024 t
+template <int R>
+__device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int r)
+{
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lt = tid & 255, p = (tid >> 8) * 32 + (lt & 31) + 128 * (lt >> 5);
+    return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (R * ACC_S) + (size_t)r * ACC_S + 16u * (uint32_t)p);
+}
+
+// One dedispersion unit in the accumulate form. hsum/hcnt: the LDS fold histogram (zero on entry and
+// exit); pw_lds: the FP16-scale word (zero on entry and exit).
+// dl holds this unit's first delay words on entry and, on return, those of the next unit (DM dn, when
+// next): each transform's delays are loaded one transform ahead.
+template <int R, bool RING32, int VAR>
+__device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 *lds_all, const AccTw &tw,
+                                         double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn)
+{
+    using A = AccShape<R>;
+    constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
+    constexpr bool f16 = (VAR & VAR_F16) != 0;
+    constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
+    const bool fold = a.fold_sum != nullptr;
+    const int gm = (int)((a.g0 + b) % a.FBS);
+    const int FBS = a.FBS;
+    const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
+    const float2 *__restrict__ ring = a.ring_p;
+    float power = 0.f;
+    float2 acc0[R == 1 ? 8 : 16], acc1[R == 4 ? 16 : 1];
+#pragma unroll
+    for (int e = 0; e < (R == 1 ? 8 : 16); ++e) acc0[e] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int e = 0; e < (R == 4 ? 16 : 1); ++e) acc1[e] = make_float2(0.f, 0.f);
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        // per-thread indices from an opaque thread index, so that their addresses are formed here and
+        // not hoisted out of the loop (where they would stay live beside the accumulators)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int sub = tid >> 8, lt = tid & 255;
+        const int col = lt & 31, jtA = lt >> 5; // pass A: column k1 = 32 sub + col, rows jtA + 8 t
+        const int jtB = lt & 7, rowB = lt >> 3; // pass B: row n2 = 32 sub + rowB, columns jtB + 8 t
+        const int k1 = sub * 32 + col, n2B = sub * 32 + rowB;
+        float2 *tile = lds_all + sub * TileWords<ACC_N>::V;
+        // gather of C_r's inputs: bins R (k1 + 128 (jtA + 8 t)) + r at positions r S + k1 + 128 jtA + 1024 t
+        // gather: the ring values load straight into the transform's registers, the chirp follows in two
+        // halves of 8 values (register budget: the accumulators stay live), and the next transform's
+        // delays are requested once this one's ring loads are out
+        float2 v[16];
+        {
+            const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
+            const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const uint32_t p = p0 + 1024u * t;
+                int src = gm - (int)((dw[t >> 2] >> (8 * (t & 3))) & 255u); // delays are < FBS: one wrap at most
+                if (src < 0) src += FBS;
+                v[t] = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + p) : ring[(size_t)src * L + p];
+            }
+            if (r + 1 < R)
+                dl = acc_delays<R>(a, d, r + 1);
+            else if (next)
+                dl = acc_delays<R>(a, dn, 0);
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                __builtin_amdgcn_sched_barrier(0);
+                float2 h[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) h[t] = ld32(Hd, p0 + 1024u * (8 * hh + t));
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    float2 &y = v[8 * hh + t];
+                    y = cmul(y, h[t]);
+                    if constexpr (f16) power += y.x * y.x + y.y * y.y;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // pass A: columns over k2, then W_L^{n2 (R k1 + r)} (the four-step twiddle times W_L^{r n2})
+        subfft<ACC_N, 32, true, +1>(v, tile, col, jtA, tw.t128);
+        {
+            const int kk = R * k1 + r;
+            const float2 c = acc_wl(tw, 16 * kk); // n2 steps by 16 along t
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                float2 w = acc_wl(tw, (jtA + 8 * q) * kk);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    v[q * 8 + t] = cmul(v[q * 8 + t], w);
+                    if (t + 1 < 8) w = cmul(w, c);
+                }
+            }
+        }
+        __syncthreads(); // every tile's exchange reads are done: the area becomes the transpose
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) lds_all[(jtA + 8 * q + 16 * t) * ACC_XS + k1] = v[q * 8 + t];
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = lds_all[n2B * ACC_XS + jtB + 8 * t];
+        __syncthreads(); // the transpose is read: the area becomes the tile exchanges again
+        // pass B: rows over k1; element (q, t) is n1 = jtB + 8 q + 16 t, j = n2B + 128 n1
+        subfft<ACC_N, 32, false, +1>(v, tile, rowB, jtB, tw.t128);
+        // accumulate T_r[j] = W_L^{128 r n1} C'_r[j] (the factor W_L^{r n2} rode in pass A's twiddle):
+        //   R = 2: acc0 += (-1)^r T;  R = 4: acc0 += (-1)^r T, acc1 += (-i)^r T;  R = 1: keep n1 >= 64.
+        // One branch-free path for every r (r is wave-uniform): the factors come from the LDS table.
+        if constexpr (R == 1) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int t = 4; t < 8; ++t) acc0[q * 4 + t - 4] = v[q * 8 + t];
+        } else {
+            const float sg = (r & 1) ? -1.f : 1.f;                          // (-1)^r
+            const float cr = (r & 1) ? 0.f : (r & 2) ? -1.f : 1.f;          // (-i)^r = cr + i sr
+            const float sr = (r & 1) ? ((r & 2) ? 1.f : -1.f) : 0.f;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int e = q * 8 + t;
+                    const float2 y = cmul(v[e], acc_wl(tw, 128 * r * (jtB + 8 * q + 16 * t)));
+                    acc0[e].x += sg * y.x;
+                    acc0[e].y += sg * y.y;
+                    if constexpr (R == 4) {
+                        acc1[e].x += cr * y.x - sr * y.y;
+                        acc1[e].y += cr * y.y + sr * y.x;
+                    }
+                }
+        }
+        __syncthreads(); // the tile exchanges are done before the next gather's transform uses them
+    }
+
+    // outputs: element e of acc0 is sample m = j (R = 2, 4) or j - S/2 (R = 1), acc1 is m = j + S
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int jtB = tid & 7, n2B = (tid >> 8) * 32 + ((tid & 255) >> 3);
+    float inv_scale = 1.f;
+    if constexpr (f16) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, 64);
+        if ((tid & 63) == 0) atomicAdd(pw_lds, power);
+        __syncthreads();
+        const float rms = sqrtf(*pw_lds);
+        int e2 = 0;
+        (void)frexpf(rms, &e2);
+        const float out_scale = rms > 0.f ? ldexpf(1.f, e2 - 1) : 1.f;
+        inv_scale = 1.f / out_scale;
+        if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
+    }
+    float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * M : nullptr;
+    uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * M : nullptr;
+    uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
+    const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * M : nullptr;
+    float *fb = reinterpret_cast<float *>(lds_all); // fold buffer: sample m at (m % MPT) FOLD_X + m / MPT
+    auto out1 = [&](int m, float2 y) {
+        if (oc) st32(oc, (uint32_t)m, y);
+        if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
+        if constexpr (f16)
+            if (oh) st32(oh, (uint32_t)m, enc_f16_pair(y, inv_scale));
+        if (fold) {
+            // primal: |y|^2; tangent: 2 Re(conj(ref) y), NaN where the primal sample is skipped (|ref| = 0)
+            float v = y.x * y.x + y.y * y.y;
+            if constexpr (tangent) {
+                const float2 rf = ld32(tref, (uint32_t)m);
+                const float I = rf.x * rf.x + rf.y * rf.y;
+                v = I != 0.0f ? 2.f * (rf.x * y.x + rf.y * y.y) : __builtin_nanf("");
+            }
+            fb[(m % MPT) * A::FOLD_X + m / MPT] = v;
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int j = n2B + 128 * (jtB + 8 * q + 16 * t);
+            if constexpr (R == 1) {
+                if (t >= 4) out1(j - S / 2, acc0[q * 4 + t - 4]);
+            } else {
+                out1(j, acc0[q * 8 + t]);
+                if constexpr (R == 4) out1(j + S, acc1[q * 8 + t]);
+            }
+        }
+    if (fold) {
+        __syncthreads();
+        // thread tid folds samples [MPT tid, MPT tid + MPT) with runs of equal bin merged in registers
+        const uint16_t *bins = a.bins + (size_t)b * M + (size_t)MPT * tid;
+        uint16_t bv[MPT];
+#pragma unroll
+        for (int i = 0; i < MPT / 8; ++i) {
+            const uint4 w4 = *reinterpret_cast<const uint4 *>(bins + 8 * i);
+            const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                bv[8 * i + 2 * k] = (uint16_t)(w[k] & 0xFFFFu);
+                bv[8 * i + 2 * k + 1] = (uint16_t)(w[k] >> 16);
+            }
+        }
+        int cur = -1;
+        double csum = 0.0;
+        unsigned int ccnt = 0;
+#pragma unroll
+        for (int i = 0; i < MPT; ++i) {
+            const float v = fb[i * A::FOLD_X + tid];
+            const bool keep = tangent ? !__builtin_isnan(v) : v != 0.0f;
+            if (keep) {
+                const int bin = (int)bv[i];
+                if (bin != cur) {
+                    if (ccnt) {
+                        atomicAdd(&hsum[cur], csum);
+                        atomicAdd(&hcnt[cur], ccnt);
+                    }
+                    cur = bin;
+                    csum = 0.0;
+                    ccnt = 0;
+                }
+                csum += (double)v;
+                ccnt += 1;
+            }
+        }
+        if (ccnt) {
+            atomicAdd(&hsum[cur], csum);
+            atomicAdd(&hcnt[cur], ccnt);
+        }
+    }
+    __syncthreads();
+    if (f16 && tid == 0) *pw_lds = 0.f;
+    if (fold) {
+        fold_flush(a, d, hsum, hcnt, tid, WG);
+        __syncthreads();
+    }
+}
+
+} // namespace msosm

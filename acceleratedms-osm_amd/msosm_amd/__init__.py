@@ -53,6 +53,7 @@ LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
 ORDER_XCD_TILE16, ORDER_XCD_TILE2 = 5, 6
 SCHEDULE_SPLIT = 8  # flag on the order: role-split launch (FFT lengths 2^14 and 2^16, plain epilogue)
+SCHEDULE_ACCUM = 16  # flag on the order: accumulate launch (FFT lengths 2^14..2^16, no scratch slot)
 
 
 class MsosmError(RuntimeError):

@@ -27,15 +27,17 @@ for p in (ROOT, os.path.join(ROOT, "acceleratedms-osm_amd")):
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 ORDERS = ["xcd_dm_major", "linear", "xcd_block_major", "xcd_tile4", "xcd_tile8", "xcd_tile16", "xcd_tile2"]
 SPLIT = 8  # schedule flag: role-split launch (MSOSM_SCHEDULE_SPLIT)
+ACCUM = 16  # schedule flag: accumulate launch (MSOSM_SCHEDULE_ACCUM)
+FORMS = {"split": SPLIT, "accum": ACCUM}
 
 
 def order_name(order: int) -> str:
-    return ORDERS[order & 7] + ("+split" if order & SPLIT else "")
+    return ORDERS[order & 7] + ("+accum" if order & ACCUM else "+split" if order & SPLIT else "")
 
 
 def order_code(name: str) -> int:
-    base, _, split = name.partition("+")
-    return ORDERS.index(base) | (SPLIT if split == "split" else 0)
+    base, _, form = name.partition("+")
+    return ORDERS.index(base) | FORMS.get(form, 0)
 
 # Uneven grid (--workload uneven): complex64 baseband, 2^26 samples at 32 MS/s, band 784-816 MHz,
 # 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
@@ -108,7 +110,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
     ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
-    ap.add_argument("--order", choices=ORDERS + [o + "+split" for o in ORDERS], default=None,
+    ap.add_argument("--order", choices=ORDERS + [o + "+" + f for o in ORDERS for f in FORMS], default=None,
                     help="fixed unit order instead of the autotuner (profiling runs)")
     ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--push-blocks", type=int, default=0,
@@ -256,6 +258,8 @@ def main():
 
 def dedispersion_kernel(order: int, fft_len: int):
     """(file stem, description) of the kernel a one-length dedispersion launch runs."""
+    if order & ACCUM and fft_len in (1 << 14, 1 << 15, 1 << 16):
+        return "acc_kernel", f"acc_kernel<{fft_len >> 14}>"
     if order & SPLIT:
         n = int(round(fft_len ** 0.5))
         return "inv_rs_kernel", f"inv_rs_kernel<{n}>"

@@ -171,12 +171,18 @@ enum {
      * of one unit while the other half runs the second pass of the previous unit on the same scratch
      * slot (FFT lengths 2^14 and 2^16, plain epilogue: complex64/uint16 output and/or fused fold;
      * other plans run the sequential form) */
-    MSOSM_SCHEDULE_SPLIT = 8
+    MSOSM_SCHEDULE_SPLIT = 8,
+    /* flag on the order: accumulate launch (FFT lengths 2^14, 2^15 and 2^16, every epilogue), where a
+     * unit's inverse transform is split by decimation in time into 1, 2 or 4 transforms of 2^14 points
+     * that stay in the CU, the kept outputs accumulated in registers: no scratch slot round trip.
+     * Takes precedence over MSOSM_SCHEDULE_SPLIT; other lengths run the sequential form */
+    MSOSM_SCHEDULE_ACCUM = 16
 };
 int msosm_plan_set_schedule(msosm_plan *plan, int workgroups, int order);
 int msosm_plan_get_schedule(msosm_plan *plan, int *workgroups, int *order);
 /* Autotuner: times the dedispersion launch of every schedule candidate (unit orders x workgroup
- * counts {max, max/2}, each also with MSOSM_SCHEDULE_SPLIT where the plan supports it) over
+ * counts {max, max/2}, each also with MSOSM_SCHEDULE_SPLIT and MSOSM_SCHEDULE_ACCUM where the plan
+ * supports them) over
  * `pushes` pushes of device input (batch*M samples each, in the plan's input format; the first push
  * of each candidate is not timed), applies the fastest and returns it. The tuning pushes consume the
  * plan's state: the plan and its attached fold are reset afterwards. */
