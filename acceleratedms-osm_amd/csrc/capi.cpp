@@ -378,7 +378,7 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(tiles_lds) +
                                                " B of LDS per workgroup, device allows " + std::to_string(max_lds));
     // the accumulate form stores the delays as bytes: blocks 0..255
-    p->accum_ok = accum_supported(log2L) && p->dc <= 256;
+    p->accum_ok = accum_supported(log2L) && p->dc <= 256 && (unsigned long long)p->FBS * p->L < (1ull << 32);
     const int acc_lds = p->accum_ok ? accum_lds_bytes(log2L) : 0;
     if (p->accum_ok) {
         p->perm_rl = log2L - 14;

@@ -10,22 +10,28 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
 {
     using A = AccShape<R>;
     extern __shared__ float2 lds_all[];
+    uint32_t *slots = reinterpret_cast<uint32_t *>(lds_all + ACC_AREA + A::TW_WORDS); // two tables of 256
+    const UnitOrder o(a, gridDim.x, blockIdx.x);
+    long long s = o.first;
+    int d = 0, b = 0;
+    bool have = o.next(s, a, d, b); // uniform over the workgroup
+    if (have) acc_slot_table<R>(a, b, slots); // made visible by the twiddle load's barrier
     const AccTw tw = acc_load_twiddles<R>(lds_all, a);
     double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + A::LDS_BYTES);
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
-    // the workgroup's units in the plan's order, each unit's first delays loaded during the previous one
-    const UnitOrder o(a, gridDim.x, blockIdx.x);
-    long long s = o.first;
-    int d = 0, b = 0;
-    bool have = o.next(s, a, d, b); // uniform over the workgroup
+    // the workgroup's units in the plan's order, each unit's first delays and ring offsets prepared
+    // during the previous one
     uint4 dl = have ? acc_delays<R>(a, d, 0) : make_uint4(0u, 0u, 0u, 0u);
+    int cur = 0;
 #pragma unroll 1
     while (have) {
         int dn = 0, bn = 0;
         const bool next = o.next(s, a, dn, bn);
-        acc_unit<R, RING32, VAR>(a, d, b, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn);
+        acc_unit<R, RING32, VAR>(a, d, b, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, slots + 256 * cur,
+                                 slots + 256 * (cur ^ 1));
+        cur ^= 1;
         d = dn;
         b = bn;
         have = next;

@@ -37,7 +37,8 @@ struct AccShape {
     static constexpr int L = R * ACC_S, M = L / 2, MPT = M / WG; // kept outputs per thread
     static constexpr int FOLD_X = WG + 1;                        // fold buffer row stride (floats)
     static constexpr int TW_WORDS = 128 + 256 + 256;             // t128, lo (<= 256), hi (<= 256)
-    static constexpr int LDS_BYTES = ACC_AREA * 8 + TW_WORDS * 8;
+    static constexpr int SLOT_BYTES = 2 * 256 * 4;               // two ring-offset tables (this unit, next)
+    static constexpr int LDS_BYTES = ACC_AREA * 8 + TW_WORDS * 8 + SLOT_BYTES;
     static_assert(MPT * FOLD_X <= ACC_AREA * 2, "fold buffer fits");
 };
 
@@ -59,6 +60,17 @@ __device__ __forceinline__ AccTw acc_load_twiddles(float2 *lds_all, const InvArg
     return AccTw{t, lo, hi, s};
 }
 
+// Ring offsets of block b's unit: tab[dd] = ((g0 + b - dd) mod FBS) L for delays dd < 256 (threads 0..255)
+template <int R>
+__device__ __forceinline__ void acc_slot_table(const InvArgs &a, int b, uint32_t *tab)
+{
+    if (threadIdx.x < 256) {
+        int sl = (int)((a.g0 + b - (long long)threadIdx.x) % a.FBS);
+        if (sl < 0) sl += a.FBS;
+        tab[threadIdx.x] = (uint32_t)sl * (uint32_t)(R * ACC_S);
+    }
+}
+
 __device__ __forceinline__ float2 acc_wl(const AccTw &tw, int m)
 {
     return cmul(tw.hi[m >> tw.s], tw.lo[m & ((1 << tw.s) - 1)]);
@@ -77,19 +89,19 @@ __device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int r)
 
 // One dedispersion unit in the accumulate form. hsum/hcnt: the LDS fold histogram (zero on entry and
 // exit); pw_lds: the FP16-scale word (zero on entry and exit).
-// dl holds this unit's first delay words on entry and, on return, those of the next unit (DM dn, when
-// next): each transform's delays are loaded one transform ahead.
+// dl holds this unit's first delay words on entry and, on return, those of the next unit (DM dn, block
+// bn, when next): each transform's delays are loaded one transform ahead. slot: this unit's ring offsets
+// (acc_slot_table) on entry; the next unit's are written to slot_next before the unit's last barrier.
 template <int R, bool RING32, int VAR>
 __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 *lds_all, const AccTw &tw,
-                                         double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn)
+                                         double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn,
+                                         int bn, const uint32_t *slot, uint32_t *slot_next)
 {
     using A = AccShape<R>;
     constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
     const bool fold = a.fold_sum != nullptr;
-    const int gm = (int)((a.g0 + b) % a.FBS);
-    const int FBS = a.FBS;
     const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring_p;
     float power = 0.f;
@@ -120,9 +132,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const uint32_t p = p0 + 1024u * t;
-                int src = gm - (int)((dw[t >> 2] >> (8 * (t & 3))) & 255u); // delays are < FBS: one wrap at most
-                if (src < 0) src += FBS;
-                v[t] = RING32 ? ld32(ring, (uint32_t)src * (uint32_t)L + p) : ring[(size_t)src * L + p];
+                const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p;
+                v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
             }
             if (r + 1 < R)
                 dl = acc_delays<R>(a, d, r + 1);
@@ -178,15 +189,21 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
 #pragma unroll
                 for (int t = 4; t < 8; ++t) acc0[q * 4 + t - 4] = v[q * 8 + t];
         } else {
-            const float sg = (r & 1) ? -1.f : 1.f;                          // (-1)^r
-            const float cr = (r & 1) ? 0.f : (r & 2) ? -1.f : 1.f;          // (-i)^r = cr + i sr
+            // one path for every r (a branch on r here costs the register allocator dearly): at r = 0 the
+            // factors are 1
+            const float sg = (r & 1) ? -1.f : 1.f;                 // (-1)^r
+            const float cr = (r & 1) ? 0.f : (r & 2) ? -1.f : 1.f; // (-i)^r = cr + i sr
             const float sr = (r & 1) ? ((r & 2) ? 1.f : -1.f) : 0.f;
+            // W_L^{128 r n1} for n1 = jtB + 8 q + 16 t: a per-thread start and two wave-uniform steps
+            // (chains of at most 8 products)
+            const float2 cq = acc_wl(tw, 1024 * r), ct = acc_wl(tw, 2048 * r);
+            float2 w = acc_wl(tw, 128 * r * jtB);
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+            for (int t = 0; t < 8; ++t) {
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
+                for (int q = 0; q < 2; ++q) {
                     const int e = q * 8 + t;
-                    const float2 y = cmul(v[e], acc_wl(tw, 128 * r * (jtB + 8 * q + 16 * t)));
+                    const float2 y = cmul(v[e], q ? cmul(w, cq) : w);
                     acc0[e].x += sg * y.x;
                     acc0[e].y += sg * y.y;
                     if constexpr (R == 4) {
@@ -194,6 +211,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                         acc1[e].y += cr * y.y + sr * y.x;
                     }
                 }
+                if (t + 1 < 8) w = cmul(w, ct);
+            }
         }
         __syncthreads(); // the tile exchanges are done before the next gather's transform uses them
     }
@@ -292,10 +311,9 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     }
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f;
-    if (fold) {
-        fold_flush(a, d, hsum, hcnt, tid, WG);
-        __syncthreads();
-    }
+    if (next) acc_slot_table<R>(a, bn, slot_next);
+    if (fold) fold_flush(a, d, hsum, hcnt, tid, WG);
+    __syncthreads();
 }
 
 } // namespace msosm

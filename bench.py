@@ -229,7 +229,7 @@ def main():
 
     # outside the timed region: the timed folds of sampled trials against the sequential launch of
     # the same design over the same pushes (identical counts; sums equal up to FP64 atomic order)
-    check = timed_output_check(ms, design, data, pushes, args.warmup + args.steps, sums, cnts)
+    check = timed_output_check(ms, design, data, pushes, args.warmup + args.steps, sums, cnts, order)
     log(f"timed-output check: {check}")
     rel_err = {"dm_grid_int8": rel_err_dm_grid(ms, design, data)}
     log(f"rel err dm grid: {rel_err['dm_grid_int8']}")
@@ -284,8 +284,11 @@ def read_traffic(kname: str, workload: str, schedule: str) -> dict:
             "traffic_source": "profiles/" + t.get("source", "")}
 
 
-def timed_output_check(ms, design, data, pushes: int, records: int, sums, cnts, ntrials: int = 8) -> dict:
-    """Fold the same pushes through a plan of sampled trials on the sequential launch (linear order)."""
+def timed_output_check(ms, design, data, pushes: int, records: int, sums, cnts, order: int, ntrials: int = 8) -> dict:
+    """Fold the same pushes through a plan of sampled trials in the linear unit order, in the timed launch's
+    arithmetic (the accumulate form, or the two-pass form that the role split shares with the sequential
+    launch): the series bits do not depend on the schedule, so the counts must be identical and the sums
+    equal up to the order of their FP64 atomic additions."""
     import numpy as np
 
     W = WORKLOAD
@@ -296,7 +299,8 @@ def timed_output_check(ms, design, data, pushes: int, records: int, sums, cnts, 
     plan = ms.Plan.from_design(sub, W["batch"], in_format=ms.IN_I8, out_flags=ms.OUT_NONE, device=0)
     fold = ms.Fold(W["period_s"], W["fs_hz"], per, W["nbins"], num_series=ntrials, device=0)
     plan.attach_fold(fold)
-    plan.set_schedule(0, ms.ORDER_LINEAR)
+    ref_order = ms.ORDER_LINEAR | (order & ACCUM)
+    plan.set_schedule(0, ref_order)
     base = data.data_ptr()
     for _ in range(records):
         for c in range(pushes):
@@ -309,7 +313,8 @@ def timed_output_check(ms, design, data, pushes: int, records: int, sums, cnts, 
     if not counts_equal or rel > 1e-9:
         raise AssertionError(f"timed launch disagrees with the sequential launch: counts equal {counts_equal}, "
                              f"max sum rel diff {rel:.3e}")
-    return {"trials": [float(d) for d in design.dms[pick]], "records": records, "reference": "sequential linear launch",
+    return {"trials": [float(d) for d in design.dms[pick]], "records": records,
+            "reference": "linear unit order, " + ("accumulate form" if order & ACCUM else "sequential two-pass form"),
             "fold_counts_equal": counts_equal, "max_fold_sum_rel_diff": rel}
 
 
