@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -248,6 +249,64 @@ extern "C" int msosm_fold_reset(msosm_fold *f)
     HIPCHK(hipMemset(f->sums, 0, n * sizeof(double)));
     HIPCHK(hipMemset(f->cnts, 0, n * sizeof(unsigned long long)));
     f->t_start = 0.0;
+    return MSOSM_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// fold without allocation: the partial sums and counts in a static device array
+// ------------------------------------------------------------------------------------------------
+// [0, MAX) FP64 sums, [MAX, 2 MAX) u64 counts. Each GPU holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns on that device.
+static __device__ __align__(256) unsigned long long scratch[2 * MSOSM_FOLD_NOALLOC_MAX_ENTRIES];
+
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr; // per device, from hipGetSymbolAddress with that device current
+
+// the device address of `scratch` on the current device (looked up once per device)
+static int scratch_address(int device, unsigned long long **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(MSOSM_ERR_HIP, std::string("fold_series_noalloc: hipGetSymbolAddress: ") +
+                                                hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = static_cast<unsigned long long *>(g_scratch_ptr[device]);
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_fold_series_noalloc(const void *pol1, const void *pol2, long long size, long long stride,
+                                         int num_series, unsigned long nbins, double t_start, double fs,
+                                         double period, float *profile, unsigned long long *counts, void *stream)
+{
+    if (!pol1 || !profile || size <= 0 || num_series <= 0 || nbins == 0 || !(fs > 0) || !(period > 0))
+        return set_error(MSOSM_ERR_ARG, "fold_series_noalloc: need series, a profile, size > 0, num_series > 0, "
+                                        "nbins > 0, fs > 0 and period > 0");
+    if (num_series > 1 && stride < size)
+        return set_error(MSOSM_ERR_ARG, "fold_series_noalloc: series stride shorter than the series size");
+    const unsigned long long n = (unsigned long long)nbins * (unsigned long long)num_series;
+    if (n > MSOSM_FOLD_NOALLOC_MAX_ENTRIES)
+        return set_error(MSOSM_ERR_ARG, "fold_series_noalloc: num_series * nbins = " + std::to_string(n) +
+                                            " exceeds the static scratch of " +
+                                            std::to_string(MSOSM_FOLD_NOALLOC_MAX_ENTRIES) + " bins");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    unsigned long long *ptr = nullptr;
+    if (scratch_address(dev, &ptr) != MSOSM_OK) return MSOSM_ERR_HIP;
+    double *sums = reinterpret_cast<double *>(ptr);
+    unsigned long long *cnts = ptr + MSOSM_FOLD_NOALLOC_MAX_ENTRIES;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the previous call's partials are cleared by a fill kernel (no memset node under graph capture)
+    HIPCHK(launch_fill(sums, (long long)(n * sizeof(double)), 0u, st));
+    HIPCHK(launch_fill(cnts, (long long)(n * sizeof(unsigned long long)), 0u, st));
+    // Fold_GPU::fold_data_phase (fold_gpu.cpp:48-54) with dt = 1/fs, then computeFoldedData (:56-60)
+    HIPCHK(launch_fold_series(static_cast<const float2 *>(pol1), static_cast<const float2 *>(pol2), size, stride,
+                              num_series, nbins, t_start, 1.0 / fs, period, sums, cnts, st));
+    HIPCHK(launch_profile(sums, cnts, profile, (long long)n, st, counts));
     return MSOSM_OK;
 }
 

@@ -136,8 +136,9 @@ hipError_t launch_bins(uint16_t *bins, long long n, unsigned long nbins, double 
 hipError_t launch_fold_series(const float2 *a, const float2 *b, long long size, long long stride, int nseries,
                               unsigned long nbins, double t_start, double dt, double period, double *sums,
                               unsigned long long *cnts, hipStream_t st);
+// profile = sum / count (0 where empty); with cnt_out, the counts are copied there too
 hipError_t launch_profile(const double *sums, const unsigned long long *cnts, float *prof, long long n,
-                          hipStream_t st);
+                          hipStream_t st, unsigned long long *cnt_out = nullptr);
 hipError_t launch_fill(void *p, long long nbytes, uint32_t value, hipStream_t st);
 
 } // namespace msosm

@@ -186,10 +186,14 @@ __global__ void fold_series_kernel(const float2 *a, const float2 *b, long long s
 }
 
 // computeFoldedData_kernel (gpu_kernel.cu:498-508)
-__global__ void profile_kernel(const double *sums, const unsigned long long *cnts, float *prof, long long n)
+__global__ void profile_kernel(const double *sums, const unsigned long long *cnts, float *prof, long long n,
+                               unsigned long long *cnt_out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) prof[i] = cnts[i] ? (float)(sums[i] / (double)cnts[i]) : 0.0f;
+    if (i < n) {
+        prof[i] = cnts[i] ? (float)(sums[i] / (double)cnts[i]) : 0.0f;
+        if (cnt_out) cnt_out[i] = cnts[i];
+    }
 }
 
 __global__ void fill_kernel(uint32_t *p, long long nwords, uint32_t value)
@@ -333,9 +337,11 @@ hipError_t launch_fold_series(const float2 *a, const float2 *b, long long size, 
     return hipGetLastError();
 }
 
-hipError_t launch_profile(const double *sums, const unsigned long long *cnts, float *prof, long long n, hipStream_t st)
+hipError_t launch_profile(const double *sums, const unsigned long long *cnts, float *prof, long long n, hipStream_t st,
+                          unsigned long long *cnt_out)
 {
-    hipLaunchKernelGGL(profile_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sums, cnts, prof, n);
+    hipLaunchKernelGGL(profile_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sums, cnts, prof, n,
+                       cnt_out);
     return hipGetLastError();
 }
 

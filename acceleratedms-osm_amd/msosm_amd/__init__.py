@@ -47,7 +47,7 @@ EXPORTS = [
     "msosm_simulate_params", "msosm_simulate_noise", "msosm_simulate_chirp", "msosm_simulate_pulsar_block",
     "msosm_simulate_params_new", "msosm_simulate_noise_new", "msosm_simulate_pulsar_new",
     "msosm_simulate_params_pulse", "msosm_simulate_pulsar",
-    "msosm_host_alloc", "msosm_host_free",
+    "msosm_host_alloc", "msosm_host_free", "msosm_fold_series_noalloc",
 ]
 LAUNCH_PERSISTENT, LAUNCH_PER_DM, LAUNCH_PER_CLASS = 0, 1, 2
 ORDER_XCD_DM_MAJOR, ORDER_LINEAR, ORDER_XCD_BLOCK_MAJOR, ORDER_XCD_TILE4, ORDER_XCD_TILE8 = 0, 1, 2, 3, 4
@@ -109,6 +109,8 @@ def lib():
                                             C.c_int),
         "msosm_plan_create": ([C.POINTER(PlanDesc), C.POINTER(vp)], C.c_int),
         "msosm_plan_destroy": ([vp], C.c_int),
+        "msosm_fold_series_noalloc": ([vp, vp, C.c_longlong, C.c_longlong, C.c_int, C.c_ulong, C.c_double,
+                                       C.c_double, C.c_double, vp, vp, vp], C.c_int),
         "msosm_plan_set_dm": ([vp, C.c_int, ip, fp], C.c_int),
         "msosm_plan_push": ([vp, vp, C.c_int], C.c_int),
         "msosm_plan_output_c64": ([vp, C.POINTER(vp)], C.c_int),

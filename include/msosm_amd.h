@@ -213,6 +213,24 @@ int msosm_fold_accumulate(msosm_fold *fold, const void *pol1_dev, const void *po
 /* get_folded_data (fold_gpu.cpp:56-60): profile[num_series][nbins] = sum/count (0 where empty);
  * any output pointer may be NULL */
 int msosm_fold_read(msosm_fold *fold, float *profile, double *sums, unsigned long long *counts);
+
+/* Fold without allocation. No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that forbid hipMalloc). One call does
+ * what msosm_fold_create + msosm_fold_accumulate + msosm_fold_read's profile do for one call of `size`
+ * samples starting at time t_start (dt = 1/fs): intensity of num_series series at pol1_dev +
+ * s*series_stride complex64 elements (pol2_dev != NULL: two polarisations, as msosm_fold_accumulate),
+ * FP64 sums and u64 counts per phase bin, then profile_dev[s][b] = sum/count (0 where empty) as float,
+ * and, when counts_dev != NULL, the counts. All pointers are device pointers. The sums and counts are
+ * kept in the library's static device array of MSOSM_FOLD_NOALLOC_MAX_ENTRIES bins (16 MiB, held on each
+ * GPU the library runs on until the process ends), cleared by each call: num_series*nbins beyond that is
+ * rejected (MSOSM_ERR_ARG). Kernels run on `stream` (NULL: the default stream); nothing is synchronised.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU (other streams or threads)
+ * race. Calls no allocating routine; its first call on a device looks the array up
+ * (hipGetSymbolAddress). */
+#define MSOSM_FOLD_NOALLOC_MAX_ENTRIES (1u << 20)
+int msosm_fold_series_noalloc(const void *pol1_dev, const void *pol2_dev, long long size, long long series_stride,
+                              int num_series, unsigned long nbins, double t_start, double fs, double period,
+                              float *profile_dev, unsigned long long *counts_dev, void *stream);
 int msosm_fold_reset(msosm_fold *fold);
 int msosm_fold_get_nbins(msosm_fold *fold, unsigned long *nbins);
 /* fused fold: every following push folds the power of every DM's output inside the dedispersion
