@@ -324,7 +324,16 @@ struct msosm_plan {
     int M = 0, L = 0, log2L = 0, D = 0, B = 0, dc = 0, FBS = 0, in_fmt = 0, out_flags = 0, G = 0;
     size_t in_bytes = 0;
     void *in_dev[2] = {nullptr, nullptr}; // staging buffers for host inputs
-    hipEvent_t in_ev[2] = {nullptr, nullptr};
+    hipEvent_t in_ev[2] = {nullptr, nullptr}; // staging slot read by its push's forward transform
+    // copies beside the compute stream (msosm_gpu_batch.cpp:15-27,203-245): host input is staged on
+    // in_stream, outputs leave on out_stream; the uint16 output has two buffers (push parity), so one
+    // push's device-to-host copy runs under the next push's kernels
+    hipStream_t in_stream = nullptr, out_stream = nullptr;
+    hipEvent_t in_done[2] = {nullptr, nullptr};    // staging slot filled
+    hipEvent_t out_ready[2] = {nullptr, nullptr};  // the push of that parity has written its outputs
+    hipEvent_t d2h_u16[2] = {nullptr, nullptr};    // uint16 buffer of that parity copied out
+    hipEvent_t d2h_c64 = nullptr, d2h_f16 = nullptr; // the single complex64 / FP16 buffers copied out
+    uint32_t *out_u16b[2] = {nullptr, nullptr};
     long long pushes = 0;
     float2 *hist = nullptr, *ring = nullptr, *H = nullptr, *scratch = nullptr, *out_c64 = nullptr, *tw = nullptr;
     uint16_t *delay = nullptr, *bins = nullptr;
@@ -366,13 +375,18 @@ static void plan_free(msosm_plan *p)
     if (!p) return;
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->in_stream) (void)hipStreamSynchronize(p->in_stream);
+    if (p->out_stream) (void)hipStreamSynchronize(p->out_stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
-                    p->bins, p->out_u16, p->out_f16, p->out_scale, p->Hdot, p->tan_y, p->ring_p, p->H_p, p->Hdot_p,
-                    p->delay_acc};
+                    p->bins, p->out_u16b[0], p->out_u16b[1], p->out_f16, p->out_scale, p->Hdot, p->tan_y, p->ring_p,
+                    p->H_p, p->Hdot_p, p->delay_acc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    for (hipEvent_t e : p->in_ev)
+    for (hipEvent_t e : {p->in_ev[0], p->in_ev[1], p->in_done[0], p->in_done[1], p->out_ready[0], p->out_ready[1],
+                         p->d2h_u16[0], p->d2h_u16[1], p->d2h_c64, p->d2h_f16})
         if (e) (void)hipEventDestroy(e);
+    if (p->in_stream) (void)hipStreamDestroy(p->in_stream);
+    if (p->out_stream) (void)hipStreamDestroy(p->out_stream);
     for (int k = 0; k < 2; ++k)
         for (auto &e : p->pending[k]) p->pool.push_back(e);
     for (auto &e : p->pool) {
@@ -491,6 +505,9 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
         e = dalloc(p, &p->in_dev[i], B * M * p->in_bytes);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&p->in_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->in_done[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->out_ready[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_u16[i], hipEventDisableTiming);
     }
     if (e == hipSuccess) e = dalloc(p, &p->hist, M * sizeof(float2));
     if (e == hipSuccess) e = dalloc(p, &p->ring, (size_t)p->FBS * L * sizeof(float2));
@@ -506,7 +523,14 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         if (e == hipSuccess && p->perm_rl) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
     }
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
-    if (e == hipSuccess && (p->out_flags & MSOSM_OUT_U16)) e = dalloc(p, &p->out_u16, D * B * M * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_c64, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_f16, hipEventDisableTiming);
+    // copy streams at the default priority, non-blocking like the compute stream
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->in_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->out_stream, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && e == hipSuccess && (p->out_flags & MSOSM_OUT_U16); ++i)
+        e = dalloc(p, &p->out_u16b[i], D * B * M * sizeof(uint32_t));
+    p->out_u16 = p->out_u16b[0];
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_F16)) e = dalloc(p, &p->out_f16, D * B * M * sizeof(uint32_t));
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_F16)) e = dalloc(p, &p->out_scale, D * B * sizeof(float));
     if (e == hipSuccess) e = hipMemset(p->hist, 0, M * sizeof(float2));
@@ -764,15 +788,28 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
     const size_t nsamp = (size_t)p->B * p->M;
     const void *src = input;
     const int slot = (int)(p->pushes & 1);
-    if (mode != MSOSM_INPUT_DEVICE) {
+    if (mode == MSOSM_INPUT_HOST_SYNC) {
         // the staging buffer was last read by the push two calls ago; wait for that to finish
         HIPCHK(hipEventSynchronize(p->in_ev[slot]));
-        if (mode == MSOSM_INPUT_HOST_SYNC)
-            HIPCHK(hipMemcpy(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice));
-        else
-            HIPCHK(hipMemcpyAsync(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipMemcpy(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice));
+        src = p->in_dev[slot];
+    } else if (mode == MSOSM_INPUT_HOST_ASYNC) {
+        // on the copy-in stream, once the push two calls ago has read the slot: the copy runs under the
+        // previous push's kernels, and this push's transforms wait for it
+        HIPCHK(hipStreamWaitEvent(p->in_stream, p->in_ev[slot], 0));
+        HIPCHK(hipMemcpyAsync(p->in_dev[slot], input, nsamp * p->in_bytes, hipMemcpyHostToDevice, p->in_stream));
+        HIPCHK(hipEventRecord(p->in_done[slot], p->in_stream));
+        HIPCHK(hipStreamWaitEvent(p->stream, p->in_done[slot], 0));
         src = p->in_dev[slot];
     }
+    // this push's uint16 buffer (parity), free once its previous copy-out (two pushes ago) is done; the
+    // single complex64 and FP16 buffers once their last copy-out is done
+    if (p->out_u16b[0]) {
+        p->out_u16 = p->out_u16b[p->pushes & 1];
+        HIPCHK(hipStreamWaitEvent(p->stream, p->d2h_u16[p->pushes & 1], 0));
+    }
+    if (p->out_c64) HIPCHK(hipStreamWaitEvent(p->stream, p->d2h_c64, 0));
+    if (p->out_f16) HIPCHK(hipStreamWaitEvent(p->stream, p->d2h_f16, 0));
     InvArgs ia{};
     if (plan_push_prepare(p, src, &ia) != MSOSM_OK) return MSOSM_ERR_HIP;
     if (mode != MSOSM_INPUT_DEVICE) HIPCHK(hipEventRecord(p->in_ev[slot], p->stream));
@@ -833,6 +870,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
         HIPCHK(hipEventRecord(ev.b, p->stream));
         p->pending[1].push_back(ev);
     }
+    HIPCHK(hipEventRecord(p->out_ready[p->pushes & 1], p->stream));
     plan_push_commit(p);
     if (tfold) tfold->t_start += static_cast<double>(nsamp) * tfold->dt;
     return MSOSM_OK;
@@ -862,8 +900,8 @@ extern "C" int msosm_plan_output_c64(msosm_plan *p, void **ptr)
 extern "C" int msosm_plan_output_u16(msosm_plan *p, void **ptr)
 {
     if (!p || !ptr) return set_error(MSOSM_ERR_ARG, "plan_output_u16: null argument");
-    if (!p->out_u16) return set_error(MSOSM_ERR_STATE, "plan_output_u16: plan has no uint16 output");
-    *ptr = p->out_u16;
+    if (!p->out_u16b[0]) return set_error(MSOSM_ERR_STATE, "plan_output_u16: plan has no uint16 output");
+    *ptr = p->out_u16b[p->pushes ? (p->pushes - 1) & 1 : 0]; // the last push's buffer
     return MSOSM_OK;
 }
 
@@ -879,22 +917,39 @@ extern "C" int msosm_plan_output_f16(msosm_plan *p, void **data, void **scales)
 extern "C" int msosm_plan_get_output(msosm_plan *p, int which, void *host)
 {
     if (!p || !host) return set_error(MSOSM_ERR_ARG, "plan_get_output: null argument");
+    if (p->pushes == 0) return set_error(MSOSM_ERR_STATE, "plan_get_output: nothing pushed yet");
     const size_t n = (size_t)p->D * p->B * p->M;
     HIPCHK(hipSetDevice(p->device));
+    // the last push's outputs, on the copy-out stream (get_output, msosm_gpu_batch.cpp:238-245)
+    const int last = (int)((p->pushes - 1) & 1);
+    hipStream_t os = p->out_stream;
     if (which == MSOSM_OUT_C64) {
         if (!p->out_c64) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no complex64 output");
-        HIPCHK(hipMemcpyAsync(host, p->out_c64, n * sizeof(float2), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamWaitEvent(os, p->out_ready[last], 0));
+        HIPCHK(hipMemcpyAsync(host, p->out_c64, n * sizeof(float2), hipMemcpyDeviceToHost, os));
+        HIPCHK(hipEventRecord(p->d2h_c64, os));
     } else if (which == MSOSM_OUT_U16) {
-        if (!p->out_u16) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no uint16 output");
-        HIPCHK(hipMemcpyAsync(host, p->out_u16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+        if (!p->out_u16b[0]) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no uint16 output");
+        HIPCHK(hipStreamWaitEvent(os, p->out_ready[last], 0));
+        HIPCHK(hipMemcpyAsync(host, p->out_u16b[last], n * sizeof(uint32_t), hipMemcpyDeviceToHost, os));
+        HIPCHK(hipEventRecord(p->d2h_u16[last], os));
     } else if (which == MSOSM_OUT_F16) {
         if (!p->out_f16) return set_error(MSOSM_ERR_STATE, "plan_get_output: plan has no FP16 output");
-        HIPCHK(hipMemcpyAsync(host, p->out_f16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamWaitEvent(os, p->out_ready[last], 0));
+        HIPCHK(hipMemcpyAsync(host, p->out_f16, n * sizeof(uint32_t), hipMemcpyDeviceToHost, os));
         HIPCHK(hipMemcpyAsync(static_cast<char *>(host) + n * sizeof(uint32_t), p->out_scale,
-                              (size_t)p->D * p->B * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+                              (size_t)p->D * p->B * sizeof(float), hipMemcpyDeviceToHost, os));
+        HIPCHK(hipEventRecord(p->d2h_f16, os));
     } else {
         return set_error(MSOSM_ERR_ARG, "plan_get_output: which must be MSOSM_OUT_C64, MSOSM_OUT_U16 or MSOSM_OUT_F16");
     }
+    return MSOSM_OK;
+}
+
+extern "C" int msosm_plan_get_output_stream(msosm_plan *p, void **stream)
+{
+    if (!p || !stream) return set_error(MSOSM_ERR_ARG, "plan_get_output_stream: null argument");
+    *stream = p->out_stream;
     return MSOSM_OK;
 }
 
@@ -902,7 +957,9 @@ extern "C" int msosm_plan_synchronize(msosm_plan *p)
 {
     if (!p) return set_error(MSOSM_ERR_ARG, "plan_synchronize: null plan");
     HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamSynchronize(p->in_stream));
     HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipStreamSynchronize(p->out_stream));
     return MSOSM_OK;
 }
 

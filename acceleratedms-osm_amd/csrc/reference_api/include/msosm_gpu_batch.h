@@ -45,6 +45,8 @@ public:
     // fold every DM's power inside the dedispersion kernel (Fold_GPU built with num_series == numDMs)
     void attach_fold(Fold_GPU *fold);
     void *get_stream();
+    // the stream get_output copies on (the reference's output_stream)
+    void *get_output_stream();
     msosm_plan *plan() { return plan_; }
     int batch_size() const { return batch; }
     std::vector<Segment> build_segments(const int *delay_block, int fft_len);

@@ -48,7 +48,7 @@ void MSOSM_DataSave_Uring::copy_to_slot()
     void *slot = nullptr;
     msosm::check(msosm_saver_acquire(saver_, &slot), "copy_to_slot");
     msosm::check(msosm_plan_get_output(plan_, MSOSM_OUT_U16, slot), "copy_to_slot");
-    msosm::check(msosm_saver_commit_stream(saver_, get_stream()), "copy_to_slot");
+    msosm::check(msosm_saver_commit_stream(saver_, get_output_stream()), "copy_to_slot");
 }
 
 void MSOSM_DataSave_Uring::poll_cuda_state()

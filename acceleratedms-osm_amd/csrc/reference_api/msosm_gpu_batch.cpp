@@ -132,6 +132,13 @@ void *MSOSM_GPU_BATCH::get_stream()
     return s;
 }
 
+void *MSOSM_GPU_BATCH::get_output_stream()
+{
+    void *s = nullptr;
+    if (plan_) msosm::check(msosm_plan_get_output_stream(plan_, &s), "get_output_stream");
+    return s;
+}
+
 void MSOSM_GPU_BATCH::attach_fold(Fold_GPU *f)
 {
     if (!plan_) msosm::check(MSOSM_ERR_STATE, "attach_fold: not initialised");

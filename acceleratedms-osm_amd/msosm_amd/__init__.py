@@ -32,7 +32,7 @@ EXPORTS = [
     "msosm_design_transfer_functions",
     "msosm_plan_create", "msosm_plan_destroy", "msosm_plan_set_dm", "msosm_plan_push", "msosm_plan_output_c64",
     "msosm_plan_output_u16", "msosm_plan_output_f16", "msosm_plan_get_output", "msosm_plan_synchronize", "msosm_plan_reset",
-    "msosm_plan_get_stream", "msosm_plan_get_info", "msosm_plan_profile", "msosm_plan_profile_read",
+    "msosm_plan_get_stream", "msosm_plan_get_output_stream", "msosm_plan_get_info", "msosm_plan_profile", "msosm_plan_profile_read",
     "msosm_fold_create", "msosm_fold_destroy", "msosm_fold_accumulate", "msosm_fold_read", "msosm_fold_reset",
     "msosm_fold_get_nbins", "msosm_plan_attach_fold", "msosm_plan_set_launch_mode",
     "msosm_design_uneven_order", "msosm_multi_create", "msosm_multi_destroy", "msosm_multi_get_info",
@@ -120,6 +120,7 @@ def lib():
         "msosm_plan_synchronize": ([vp], C.c_int),
         "msosm_plan_reset": ([vp], C.c_int),
         "msosm_plan_get_stream": ([vp, C.POINTER(vp)], C.c_int),
+        "msosm_plan_get_output_stream": ([vp, C.POINTER(vp)], C.c_int),
         "msosm_plan_get_info": ([vp, C.POINTER(PlanInfo)], C.c_int),
         "msosm_plan_profile": ([vp, C.c_int], C.c_int),
         "msosm_plan_profile_read": ([vp, C.c_int, dp, C.POINTER(C.c_longlong)], C.c_int),

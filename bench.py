@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pushes", type=int, default=0, help="pushes per step (default: the whole record)")
-    ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine"], default="dm_grid")
+    ap.add_argument("--workload", choices=["dm_grid", "uneven", "refine", "save"], default="dm_grid")
     ap.add_argument("--order", choices=ORDERS + [o + "+" + f for o in ORDERS for f in FORMS], default=None,
                     help="fixed unit order instead of the autotuner (profiling runs)")
     ap.add_argument("--refine-steps", type=int, default=20)
@@ -143,6 +143,9 @@ def main():
         return
     if args.workload == "refine":
         bench_refine(args)
+        return
+    if args.workload == "save":
+        bench_save(args, pushes=args.pushes or 6)
         return
 
     import numpy as np
@@ -241,6 +244,7 @@ def main():
     del data
     torch.cuda.empty_cache()
 
+    save = None if args.no_uneven else bench_save(args, nested=True)
     nested = None
     if not args.no_uneven:
         nested = bench_uneven(args, nested=True)
@@ -251,7 +255,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": dict(W, schedule=schedule), "roofline": roof, "cpu_baseline": cpu,
-        "rel_err_vs_fp64": rel_err, "timed_output_check": check, "uneven_grid": nested,
+        "rel_err_vs_fp64": rel_err, "timed_output_check": check, "uneven_grid": nested, "save_mode": save,
     }
     print(json.dumps(out), flush=True)
 
@@ -468,6 +472,71 @@ def rel_err_uneven(ms, design, push_blocks: int, data) -> dict:
         res[str(int(2 * M))] = {"dm": float(design.dms[d]), "rel_l2": l2, "max_abs_rel": mx}
     multi.close()
     return {"per_length": res, "rel_l2_max": max(v["rel_l2"] for v in res.values()), "samples": per, "bar": 1e-5}
+
+
+def bench_save(args, pushes: int = 6, repeats: int = 3, nested: bool = False):
+    """Save mode end to end, as the reference's speed harness times it (test_msosm_speed.cpp:143-208): the
+    grid's 512 trials with uint16-pair output, each push's int8 input copied from pinned host memory and
+    its 512 x 32 x 32768 uint16 pairs (2 GiB) copied back into one pinned host buffer, `repeats` times over
+    `pushes` pushes, mean and standard deviation of the wall time with the copies included. Pipelined:
+    input and output copies on the plan's copy streams, one synchronize per repeat (the harness's loop);
+    serialised: the same with a synchronize after every push."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    import msosm_amd as ms
+
+    W = WORKLOAD
+    f0 = W["fc_hz"] - W["bw_hz"] / 2
+    design = ms.Design(W["bw_hz"], f0, np.linspace(W["dm_lo"], W["dm_hi"], W["num_dms"]).astype(np.float32))
+    B, M, D = W["batch"], design.M, W["num_dms"]
+    per = B * M
+    lib = ms.lib()
+
+    def pinned(nbytes):
+        p = C.c_void_p()
+        ms._check(lib.msosm_host_alloc(C.c_ulonglong(nbytes), C.byref(p)))
+        return p
+
+    rng = np.random.default_rng(7)
+    q = rng.integers(-127, 128, size=2 * per * pushes, dtype=np.int8)
+    pin_in = pinned(q.nbytes)
+    C.memmove(pin_in, q.ctypes.data, q.nbytes)
+    out_bytes = D * per * 4
+    pin_out = pinned(out_bytes)
+    plan = ms.Plan.from_design(design, B, in_format=ms.IN_I8, out_flags=ms.OUT_U16, device=0)
+    plan.set_schedule(0, order_code("xcd_tile16+accum"))
+    res = {}
+    for name, sync_each in (("pipelined", False), ("serialised", True)):
+        times = []
+        for rep in range(repeats + 1):  # the first repeat warms up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for c in range(pushes):
+                ms._check(lib.msosm_plan_push(plan.h, C.c_void_p(pin_in.value + c * per * 2), ms.INPUT_HOST_ASYNC))
+                ms._check(lib.msosm_plan_get_output(plan.h, ms.OUT_U16, pin_out))
+                if sync_each:
+                    plan.synchronize()
+            plan.synchronize()
+            if rep:
+                times.append((time.perf_counter() - t0) * 1e3)
+        t = np.array(times)
+        res[name] = {"mean_ms": float(t.mean()), "std_ms": float(t.std()),
+                     "rate": D * per * pushes / (t.mean() / 1e3), "d2h_GB_per_s": out_bytes * pushes / (t.mean() / 1e3) / 1e9}
+        log(f"save mode {name}: {t.mean():.1f} +- {t.std():.1f} ms for {pushes} pushes "
+            f"({res[name]['rate']:.3e} DM-trial*samples/s, {res[name]['d2h_GB_per_s']:.1f} GB/s out)")
+    plan.close()
+    lib.msosm_host_free(pin_in)
+    lib.msosm_host_free(pin_out)
+    out = {"workload": "msosm_dm_grid_int8 save mode (uint16-pair output, host in and out)", "pushes": pushes,
+           "repeats": repeats, "output_bytes_per_push": out_bytes, "unit": "DM-trial*samples/s",
+           "schedule": "xcd_tile16+accum", **res,
+           "overlap_gain": res["serialised"]["mean_ms"] / res["pipelined"]["mean_ms"]}
+    if nested:
+        return out
+    print(json.dumps(out), flush=True)
 
 
 def synth_pulsar(n, fs, f0, dm, period, duty, amp, sigma, seed, device):

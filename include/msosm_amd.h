@@ -127,18 +127,26 @@ enum {
     MSOSM_INPUT_HOST_SYNC = 2   /* host memory copied before the call returns (fold-mode cudaMemcpy) */
 };
 /* filter_block_uint16 (msosm_gpu_batch.cpp:178-236) for any input format: batch*M new samples.
- * Host inputs go through two staging buffers, so a copy overlaps the previous push's kernels. */
+ * Host inputs go through two staging buffers. MSOSM_INPUT_HOST_ASYNC copies on the plan's copy-in
+ * stream, ordered after the push two calls back has read the staging buffer, so the copy runs under the
+ * previous push's kernels and this push's kernels wait for it (the reference's fft_stream, :203-209). */
 int msosm_plan_push(msosm_plan *plan, const void *input, int input_mode);
 /* get_output_pointer (dedispersion/include/msosm_gpu_batch.h:24): complex64 [D][B][M] device buffer */
 int msosm_plan_output_c64(msosm_plan *plan, void **dev_ptr);
-/* uint16-pair [D][B][M] device buffer (save mode) */
+/* uint16-pair [D][B][M] device buffer (save mode) of the last push: the uint16 output has two buffers
+ * that pushes alternate, so one push's copy-out overlaps the next push's kernels */
 int msosm_plan_output_u16(msosm_plan *plan, void **dev_ptr);
 /* FP16-pair [D][B][M] device buffer and its [D][B] float scales */
 int msosm_plan_output_f16(msosm_plan *plan, void **dev_data, void **dev_scales);
-/* get_output (msosm_gpu_batch.cpp:238-245): D2H copy of one output buffer, ordered after the last push.
- * For MSOSM_OUT_F16, host_dst receives the [D][B][M] pairs followed by the [D][B] float scales. */
+/* get_output (msosm_gpu_batch.cpp:238-245): D2H copy of the last push's output, on the plan's copy-out
+ * stream (the reference's output_stream) once that push's kernels are done; the push that next writes
+ * the same device buffer waits for the copy. Asynchronous for pinned host memory: msosm_plan_synchronize
+ * (or the copy-out stream) completes it. For MSOSM_OUT_F16, host_dst receives the [D][B][M] pairs
+ * followed by the [D][B] float scales. */
 int msosm_plan_get_output(msosm_plan *plan, int which, void *host_dst);
-/* synchronize (msosm_gpu_batch.cpp:247-250) */
+/* the stream msosm_plan_get_output copies on (MSOSM_DataSave_Uring records its slot events there) */
+int msosm_plan_get_output_stream(msosm_plan *plan, void **stream);
+/* synchronize (msosm_gpu_batch.cpp:247-250): the plan's compute and copy streams */
 int msosm_plan_synchronize(msosm_plan *plan);
 /* forget the stream history: zero ring + input history, block counter = 0 */
 int msosm_plan_reset(msosm_plan *plan);
