@@ -53,17 +53,26 @@ void MSOSM_DataSave_Uring::copy_to_slot()
 
 void MSOSM_DataSave_Uring::poll_cuda_state()
 {
-    msosm::check(MSOSM_ERR_STATE, "poll_cuda_state: runs inside start_saving()");
+    // msosm_data_save_uring.cpp:115-154, on the calling thread (start_saving runs it on its own)
+    if (!saver_) msosm::check(MSOSM_ERR_STATE, "poll_cuda_state: call initialize_uring first");
+    started_ = true;
+    msosm::check(msosm_saver_run_poller(saver_), "poll_cuda_state");
 }
 
-void MSOSM_DataSave_Uring::save_to_disk(int)
+void MSOSM_DataSave_Uring::save_to_disk(int save_index)
 {
-    msosm::check(MSOSM_ERR_STATE, "save_to_disk: runs inside start_saving()");
+    // :156-222: io_uring writes of one slot, on the calling thread
+    if (!saver_) msosm::check(MSOSM_ERR_STATE, "save_to_disk: call initialize_uring first");
+    started_ = true;
+    msosm::check(msosm_saver_run_writer(saver_, save_index, 1), "save_to_disk");
 }
 
-void MSOSM_DataSave_Uring::save_to_disk_pwrite(int)
+void MSOSM_DataSave_Uring::save_to_disk_pwrite(int save_index)
 {
-    msosm::check(MSOSM_ERR_STATE, "save_to_disk_pwrite: runs inside start_saving(); select it with use_pwrite(true)");
+    // :224-258: pwrite writes of one slot, on the calling thread
+    if (!saver_) msosm::check(MSOSM_ERR_STATE, "save_to_disk_pwrite: call initialize_uring first");
+    started_ = true;
+    msosm::check(msosm_saver_run_writer(saver_, save_index, 0), "save_to_disk_pwrite");
 }
 
 void MSOSM_DataSave_Uring::start_saving()

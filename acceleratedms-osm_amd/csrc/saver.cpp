@@ -161,7 +161,8 @@ struct msosm_saver {
     std::thread poller;
     std::vector<std::thread> writers;
     std::vector<Uring> rings;
-    bool started = false;
+    std::atomic<bool> started{false}; // the loops run (the library's threads, or the caller's: run_*)
+    bool own_threads = false;         // msosm_saver_start made them
     std::mutex err_mu;
     std::string err;
     int err_code = MSOSM_OK;
@@ -235,7 +236,7 @@ static int write_slot_uring(msosm_saver *s, Uring &ring, const char *base, unsig
     return first_err;
 }
 
-static void writer_loop(msosm_saver *s, int k)
+static void writer_loop(msosm_saver *s, int k, bool uring)
 {
     unsigned spins = 0;
     for (;;) {
@@ -250,7 +251,7 @@ static void writer_loop(msosm_saver *s, int k)
                 const char *base = s->slots + (size_t)k * s->slot_bytes;
                 const unsigned long long off = s->seq[k] * (unsigned long long)s->block;
                 int r = 0;
-                if (s->flags & MSOSM_SAVE_URING) {
+                if (uring) {
                     r = write_slot_uring(s, s->rings[k], base, off);
                 } else {
                     for (size_t i = 0; i < s->fds.size(); ++i) {
@@ -442,8 +443,37 @@ extern "C" int msosm_saver_start(msosm_saver *s)
     if (!s) return set_error(MSOSM_ERR_ARG, "saver_start: null saver");
     if (s->started) return set_error(MSOSM_ERR_STATE, "saver_start: already started");
     s->started = true;
+    s->own_threads = true;
     s->poller = std::thread(poll_loop, s);
-    for (int k = 0; k < s->nslots; ++k) s->writers.emplace_back(writer_loop, s, k);
+    const bool uring = (s->flags & MSOSM_SAVE_URING) != 0;
+    for (int k = 0; k < s->nslots; ++k) s->writers.emplace_back(writer_loop, s, k, uring);
+    return MSOSM_OK;
+}
+
+// poll_cuda_state (data_save/msosm_data_save_uring.cpp:115-154) on the calling thread: hands each committed
+// slot to its writer once its copy is done, then marks every slot done; returns when all save_count
+// pushes are through. Every slot's writer must run meanwhile (msosm_saver_run_writer on other threads).
+extern "C" int msosm_saver_run_poller(msosm_saver *s)
+{
+    if (!s) return set_error(MSOSM_ERR_ARG, "saver_run_poller: null saver");
+    if (s->own_threads) return set_error(MSOSM_ERR_STATE, "saver_run_poller: msosm_saver_start runs the loops");
+    s->started = true;
+    poll_loop(s);
+    return MSOSM_OK;
+}
+
+// save_to_disk / save_to_disk_pwrite (:156-258) of one slot on the calling thread, with io_uring (the saver
+// made with MSOSM_SAVE_URING) or pwrite; returns once the poller has marked the slot done. Write errors are
+// reported by msosm_saver_join.
+extern "C" int msosm_saver_run_writer(msosm_saver *s, int slot, int use_uring)
+{
+    if (!s) return set_error(MSOSM_ERR_ARG, "saver_run_writer: null saver");
+    if (slot < 0 || slot >= s->nslots) return set_error(MSOSM_ERR_ARG, "saver_run_writer: slot index out of range");
+    if (s->own_threads) return set_error(MSOSM_ERR_STATE, "saver_run_writer: msosm_saver_start runs the loops");
+    if (use_uring && !(s->flags & MSOSM_SAVE_URING))
+        return set_error(MSOSM_ERR_STATE, "saver_run_writer: io_uring writes need a saver made with MSOSM_SAVE_URING");
+    s->started = true;
+    writer_loop(s, slot, use_uring != 0);
     return MSOSM_OK;
 }
 
@@ -452,7 +482,7 @@ extern "C" int msosm_saver_acquire(msosm_saver *s, void **slot_data)
     if (!s || !slot_data) return set_error(MSOSM_ERR_ARG, "saver_acquire: null argument");
     if (s->committed >= s->save_count) return set_error(MSOSM_ERR_STATE, "saver_acquire: all pushes already saved");
     const int k = s->fill_index;
-    if (!s->started && s->state[k].load(std::memory_order_acquire) != EMPTY)
+    if (!s->started.load() && s->state[k].load(std::memory_order_acquire) != EMPTY)
         return set_error(MSOSM_ERR_STATE, "saver_acquire: every slot is full and saving has not started");
     unsigned spins = 0;
     while (s->state[k].load(std::memory_order_acquire) != EMPTY) cpu_relax(spins);
@@ -515,7 +545,7 @@ extern "C" int msosm_saver_join(msosm_saver *s)
 extern "C" int msosm_saver_destroy(msosm_saver *s)
 {
     if (!s) return MSOSM_OK;
-    if (s->started && (s->poller.joinable() || !s->writers.empty())) {
+    if (s->own_threads && (s->poller.joinable() || !s->writers.empty())) {
         // abandoned mid-run: release the threads by marking every slot done
         s->save_count.store(s->committed, std::memory_order_release);
         if (s->poller.joinable()) s->poller.join();

@@ -43,6 +43,7 @@ EXPORTS = [
     "msosm_multi_set_graph",
     "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
+    "msosm_saver_run_poller", "msosm_saver_run_writer",
     "msosm_plan_set_schedule", "msosm_plan_get_schedule", "msosm_plan_autotune", "msosm_stream_wait",
     "msosm_simulate_params", "msosm_simulate_noise", "msosm_simulate_chirp", "msosm_simulate_pulsar_block",
     "msosm_simulate_params_new", "msosm_simulate_noise_new", "msosm_simulate_pulsar_new",
@@ -154,6 +155,8 @@ def lib():
         "msosm_saver_create": ([C.POINTER(C.c_char_p), C.c_int, C.c_size_t, C.c_int, C.c_ulong, C.c_int,
                                 C.POINTER(vp)], C.c_int),
         "msosm_saver_start": ([vp], C.c_int),
+        "msosm_saver_run_poller": ([vp], C.c_int),
+        "msosm_saver_run_writer": ([vp, C.c_int, C.c_int], C.c_int),
         "msosm_saver_acquire": ([vp, C.POINTER(vp)], C.c_int),
         "msosm_saver_commit_stream": ([vp, vp], C.c_int),
         "msosm_saver_commit_ready": ([vp], C.c_int),
@@ -601,6 +604,14 @@ class Saver:
 
     def start(self):
         _check(lib().msosm_saver_start(self.h))
+
+    def run_poller(self):
+        """poll_cuda_state on the calling thread (instead of start): returns when every push is through."""
+        _check(lib().msosm_saver_run_poller(self.h))
+
+    def run_writer(self, slot: int, uring: bool = True):
+        """save_to_disk (uring) or save_to_disk_pwrite of one slot on the calling thread."""
+        _check(lib().msosm_saver_run_writer(self.h, slot, int(uring)))
 
     def push_host(self, data: np.ndarray):
         """Copy one push (num_files * block_bytes bytes) into the next slot and commit it."""

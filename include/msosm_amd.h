@@ -322,6 +322,13 @@ int msosm_saver_create(const char *const *paths, int num_files, size_t block_byt
                        unsigned long save_count, int flags, msosm_saver **saver);
 /* start_saving (:260-268): the poll thread and the writer threads */
 int msosm_saver_start(msosm_saver *saver);
+/* Or the caller's own threads run the loops instead of msosm_saver_start. poll_cuda_state (:115-154):
+ * hands each committed slot to its writer once its copy has completed, then marks every slot done;
+ * returns when all save_count pushes are through (every slot's writer must run meanwhile). */
+int msosm_saver_run_poller(msosm_saver *saver);
+/* save_to_disk (:156-222, use_uring != 0; the saver needs MSOSM_SAVE_URING) or save_to_disk_pwrite
+ * (:224-258) of one slot; returns once the poller has marked the slot done. */
+int msosm_saver_run_writer(msosm_saver *saver, int slot, int use_uring);
 /* copy_to_slot, first half (:103-107): wait until the next slot is free; *slot_data = its buffer */
 int msosm_saver_acquire(msosm_saver *saver, void **slot_data);
 /* copy_to_slot, second half (:108-112): the acquired slot is filled once the work queued so far on

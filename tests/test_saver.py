@@ -137,3 +137,57 @@ def test_read_complex_bin_decodes_saved_words(tmp_path):
     # start_index and length count samples (4 bytes each)
     assert np.array_equal(ms.read_complex_bin(path, 7, 5), expect[7:12])
     assert np.array_equal(ms.read_complex_bin(path, 30), np.concatenate([expect[30:], np.zeros(nblock - 36)]))
+
+
+@pytest.mark.parametrize("uring", [True, False])
+def test_caller_threads_run_the_poll_and_write_loops(tmp_path, uring):
+    # poll_cuda_state / save_to_disk / save_to_disk_pwrite (msosm_data_save_uring.cpp:115-258) run on the
+    # caller's threads instead of start_saving's: one poller, one writer per slot, the producer on the
+    # main thread; the files hold every push's block of each series at its offset
+    import threading
+
+    nfiles, block, slots, pushes = 2, 4096, 3, 7
+    paths = [str(tmp_path / f"s{i}.dat") for i in range(nfiles)]
+    sv = ms.Saver(paths, block, slots, pushes, flags=ms.SAVE_URING if uring else 0)
+    errors = []
+
+    def run(fn, *a):
+        try:
+            fn(*a)
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(sv.run_poller,))]
+    threads += [threading.Thread(target=run, args=(sv.run_writer, k, uring)) for k in range(slots)]
+    for t in threads:
+        t.start()
+    rng = np.random.default_rng(3)
+    data = rng.integers(0, 256, size=(pushes, nfiles, block), dtype=np.uint8)
+    for k in range(pushes):
+        sv.push_host(data[k])
+    for t in threads:
+        t.join(timeout=60)
+        assert not t.is_alive()
+    assert not errors, errors
+    sv.join()
+    sv.close()
+    for i, p in enumerate(paths):
+        got = np.fromfile(p, dtype=np.uint8)
+        assert np.array_equal(got, data[:, i, :].reshape(-1))
+
+
+def test_caller_loops_refuse_misuse(tmp_path):
+    sv = ms.Saver([str(tmp_path / "a.dat")], 4096, 2, 2, flags=0)
+    with pytest.raises(ms.MsosmError, match="MSOSM_SAVE_URING"):
+        sv.run_writer(0, True)
+    with pytest.raises(ms.MsosmError, match="out of range"):
+        sv.run_writer(5, False)
+    sv.close()
+    sv = ms.Saver([str(tmp_path / "b.dat")], 4096, 2, 2)
+    sv.start()
+    with pytest.raises(ms.MsosmError, match="msosm_saver_start runs the loops"):
+        sv.run_poller()
+    for k in range(2):
+        sv.push_host(np.zeros(4096, np.uint8))
+    sv.join()
+    sv.close()
