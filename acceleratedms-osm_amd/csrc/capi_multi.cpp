@@ -440,6 +440,79 @@ extern "C" int msosm_multi_push(msosm_multi *m, const void *input, int mode)
     return MSOSM_OK;
 }
 
+// every class plan and fold back to the stream start (zero history, empty folds)
+static int multi_reset(msosm_multi *m)
+{
+    for (auto &c : m->classes) {
+        int rc = msosm_plan_reset(c.plan);
+        if (rc == MSOSM_OK && c.fold) rc = msosm_fold_reset(c.fold);
+        if (rc != MSOSM_OK) return rc;
+    }
+    return MSOSM_OK;
+}
+
+// The autotuner of an uneven grid: each length class's plan tunes its own tile schedule (unit order,
+// workgroups, launch form: msosm_plan_autotune on the same device input), then the push is timed as one
+// launch over all classes and as one tuned launch per class, and the faster is kept. The tuning pushes
+// consume the stream: every class plan and fold is reset afterwards.
+extern "C" int msosm_multi_autotune(msosm_multi *m, const void *input, int pushes, int *best_mode, double *best_ms)
+{
+    if (!m || !input) return set_error(MSOSM_ERR_ARG, "multi_autotune: null argument");
+    if (pushes < 2) return set_error(MSOSM_ERR_ARG, "multi_autotune: need at least 2 pushes of input");
+    if (m->launch_mode == MSOSM_LAUNCH_PER_DM)
+        return set_error(MSOSM_ERR_STATE, "multi_autotune: tunes the persistent and per-class forms only");
+    HIPCHK_MULTI(hipSetDevice(m->device));
+    const bool graph = m->use_graph;
+    m->use_graph = false;
+    drop_graph(m);
+    for (auto &c : m->classes) {
+        int w = 0, o = 0;
+        double t = 0;
+        const int rc = msosm_plan_autotune(c.plan, input, pushes, &w, &o, &t);
+        if (rc != MSOSM_OK) return rc;
+    }
+    const size_t step = (size_t)m->push * m->in_bytes;
+    double best = 1e300;
+    int mode = MSOSM_LAUNCH_PER_CLASS;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK_MULTI(hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    int rc = e == hipSuccess ? MSOSM_OK : set_error(MSOSM_ERR_HIP, std::string("multi_autotune: ") + hipGetErrorString(e));
+    for (int cand : {MSOSM_LAUNCH_PERSISTENT, MSOSM_LAUNCH_PER_CLASS}) {
+        if (rc != MSOSM_OK) break;
+        if (cand == MSOSM_LAUNCH_PERSISTENT && !fused_eligible(m)) continue;
+        m->launch_mode = cand;
+        rc = multi_reset(m);
+        // the first push of each candidate is not timed
+        if (rc == MSOSM_OK) rc = msosm_multi_push(m, input, MSOSM_INPUT_DEVICE);
+        if (rc == MSOSM_OK && hipEventRecord(a, m->stream) != hipSuccess) rc = set_error(MSOSM_ERR_HIP, "multi_autotune: event");
+        for (int k = 1; k < pushes && rc == MSOSM_OK; ++k)
+            rc = msosm_multi_push(m, static_cast<const char *>(input) + k * step, MSOSM_INPUT_DEVICE);
+        float ms = 0;
+        if (rc == MSOSM_OK) {
+            e = hipEventRecord(b, m->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+            if (e != hipSuccess) rc = set_error(MSOSM_ERR_HIP, std::string("multi_autotune: ") + hipGetErrorString(e));
+        }
+        const double per = ms / (pushes - 1);
+        if (rc == MSOSM_OK && per < best) {
+            best = per;
+            mode = cand;
+        }
+    }
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc != MSOSM_OK) return rc;
+    m->launch_mode = mode;
+    m->use_graph = graph;
+    rc = multi_reset(m);
+    if (rc != MSOSM_OK) return rc;
+    if (best_mode) *best_mode = mode;
+    if (best_ms) *best_ms = best;
+    return MSOSM_OK;
+}
+
 extern "C" int msosm_multi_output_c64(msosm_multi *m, int dm, void **dev_ptr)
 {
     if (check_dm(m, dm) != MSOSM_OK || !dev_ptr) return set_error(MSOSM_ERR_ARG, "multi_output_c64: bad arguments");

@@ -40,7 +40,7 @@ EXPORTS = [
     "msosm_multi_get_output", "msosm_multi_fold", "msosm_multi_fold_read", "msosm_multi_synchronize",
     "msosm_multi_profile", "msosm_multi_profile_read",
     "msosm_design_transfer_function_ddm", "msosm_plan_set_dm_tangent", "msosm_plan_push_tangent",
-    "msosm_multi_set_graph",
+    "msosm_multi_set_graph", "msosm_multi_autotune",
     "msosm_saver_create", "msosm_saver_start", "msosm_saver_acquire", "msosm_saver_commit_stream",
     "msosm_saver_commit_ready", "msosm_saver_join", "msosm_saver_destroy", "msosm_saver_file_name",
     "msosm_saver_run_poller", "msosm_saver_run_writer",
@@ -152,6 +152,7 @@ def lib():
         "msosm_plan_set_dm_tangent": ([vp, C.c_int, fp], C.c_int),
         "msosm_plan_push_tangent": ([vp, vp, C.c_int, vp], C.c_int),
         "msosm_multi_set_graph": ([vp, C.c_int], C.c_int),
+        "msosm_multi_autotune": ([vp, vp, C.c_int, ip, dp], C.c_int),
         "msosm_saver_create": ([C.POINTER(C.c_char_p), C.c_int, C.c_size_t, C.c_int, C.c_ulong, C.c_int,
                                 C.POINTER(vp)], C.c_int),
         "msosm_saver_start": ([vp], C.c_int),
@@ -534,6 +535,14 @@ class Multi:
 
     def set_graph(self, enable: bool = True):
         _check(lib().msosm_multi_set_graph(self.h, int(enable)))
+
+    def autotune(self, input_dev: int, pushes: int = 3) -> tuple[int, float]:
+        """Tune every length class's schedule, then keep the faster of one launch over all classes and
+        one tuned launch per class (msosm_multi_autotune). Returns (launch mode, ms per push); resets
+        the stream."""
+        mode, t = C.c_int(), C.c_double()
+        _check(lib().msosm_multi_autotune(self.h, C.c_void_p(input_dev), pushes, C.byref(mode), C.byref(t)))
+        return mode.value, t.value
 
     def push(self, x) -> None:
         if isinstance(x, int):

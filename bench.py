@@ -408,8 +408,13 @@ def bench_uneven(args, nested: bool = False):
     pushes = N // per
     if args.pushes and not nested:
         pushes = min(pushes, args.pushes)
-    one = timed(multi, ms.LAUNCH_PERSISTENT, pushes, args.steps, graph=args.graph and not nested)
-    log(f"uneven one launch: {one['rate']:.3e} DM-trial*samples/s, {one['ms_per_push']:.3f} ms/push")
+    # each length class tunes its schedule; then one launch over all classes or one tuned launch per class
+    t = time.perf_counter()
+    mode, tms = multi.autotune(base, 3)
+    mode_name = {ms.LAUNCH_PERSISTENT: "one launch over all classes", ms.LAUNCH_PER_CLASS: "one launch per class"}[mode]
+    log(f"uneven autotune: {mode_name}, {tms:.3f} ms per push ({time.perf_counter() - t:.1f} s)")
+    one = timed(multi, mode, pushes, args.steps, graph=args.graph and not nested and mode == ms.LAUNCH_PERSISTENT)
+    log(f"uneven tuned: {one['rate']:.3e} DM-trial*samples/s, {one['ms_per_push']:.3f} ms/push")
     multi.close()
     rel = rel_err_uneven(ms, design, W["push_blocks"], data) if nested else None
     # the plain form: one launch per DM, at the harness's 32 blocks per call and at other batches
@@ -423,12 +428,14 @@ def bench_uneven(args, nested: bool = False):
     best_pb = max(per_dm, key=lambda k: per_dm[k]["rate"])
     per_sample = alg_bytes_per_dm_sample(8, 8, D, int(2 * design.M.max()), N)
     alg = D * per * per_sample
-    avg_s = one["kernel_ms"] / one["launches"] / 1e3
+    # dedispersion time per push: one launch, or one launch per class
+    avg_s = one["kernel_ms"] / one["pushes"] / 1e3
     roof = {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": alg / avg_s / HBM_PEAK, "traffic": None, "kernel": "inv_multi_kernel",
-            "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "alg_bytes_per_launch": alg,
-            "alg_bytes_per_dm_sample": per_sample}
-    if not (args.graph and not nested):
+            "frac": alg / avg_s / HBM_PEAK, "traffic": None,
+            "kernel": "inv_multi_kernel" if mode == ms.LAUNCH_PERSISTENT else "per-class launches (tuned)",
+            "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "launches_per_push": one["launches"] / one["pushes"],
+            "alg_bytes_per_launch": alg, "alg_bytes_per_dm_sample": per_sample}
+    if not (args.graph and not nested) and mode == ms.LAUNCH_PERSISTENT:
         roof.update(read_traffic("inv_multi_kernel", "msosm_uneven_c64", "one launch"))
     del data
     torch.cuda.empty_cache()
@@ -441,7 +448,9 @@ def bench_uneven(args, nested: bool = False):
     out = {
         "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s",
         "ms_per_step": one["ms_per_push"] * pushes, "steps": args.steps,
-        "config": dict(W, length_classes=lengths, graph=bool(args.graph and not nested)), "roofline": roof,
+        "config": dict(W, length_classes=lengths, graph=bool(args.graph and not nested),
+                       schedule={"launch_mode": mode_name, "autotuned": True, "ms_per_push_tuning": tms}),
+        "roofline": roof,
         "per_dm_launch_form": plain,
     }
     if nested:

@@ -291,6 +291,13 @@ int msosm_multi_set_launch_mode(msosm_multi *multi, int mode);
  * re-parameterised on later ones. Profiling (kernel 1) then times the whole push graph. */
 int msosm_multi_set_graph(msosm_multi *multi, int enable);
 int msosm_multi_push(msosm_multi *multi, const void *input, int input_mode);
+/* Autotuner of an uneven grid (no reference counterpart; BASELINE's per-length-class tuning): every
+ * length class's plan tunes its tile schedule (msosm_plan_autotune on the same device input: unit
+ * order, workgroups, role split / accumulate form where the length has them), then a push is timed as
+ * one launch over all classes (MSOSM_LAUNCH_PERSISTENT) and as one tuned launch per class
+ * (MSOSM_LAUNCH_PER_CLASS) over `pushes` pushes (the first not timed), and the faster becomes the launch
+ * mode. Every class plan and fold is reset afterwards. best_ms: time per push of the kept mode. */
+int msosm_multi_autotune(msosm_multi *multi, const void *input_dev, int pushes, int *best_mode, double *best_ms);
 /* complex64 series of one DM for the last push (push_samples values) */
 int msosm_multi_output_c64(msosm_multi *multi, int dm, void **dev_ptr);
 int msosm_multi_get_output(msosm_multi *multi, int dm, void *host_dst);

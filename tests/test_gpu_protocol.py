@@ -26,15 +26,16 @@ def fold_abs(x, fold_start, block, repeat):
 
 
 def test_osm_and_msosm_profiles_agree_over_the_common_periods():
-    bw, f0, dm, period = 16e6, 1392e6, 50.0, 0.002048
-    block = int(period * bw)                         # 32768 samples per pulse period
+    # a 1.024 ms period: 16384 samples, so the generator's chunk transform is 2^22 points
+    bw, f0, dm, period = 16e6, 1392e6, 50.0, 0.001024
+    block = int(period * bw)
     nd = ms.design_nd(bw, f0, dm)
     m_osm = 2 * (1 << (nd - 1).bit_length())         # Prepare_OSM: 2 nextpow2(Nd) (osm_prepare.cpp:17-30)
     osm = ms.Design(bw, f0, [dm], M=m_osm)
     msosm = ms.Design(bw, f0, [dm])
     assert osm.delaycount == 1 and msosm.M < m_osm
     b_osm, b_ms = 2, 32
-    repeat = 64
+    repeat = 128
     common = m_osm                                   # check_msosm.cpp:101-110
     fold_start = -(-common // block) * block
     need = fold_start + repeat * block
