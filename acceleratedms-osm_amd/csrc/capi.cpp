@@ -350,8 +350,9 @@ struct msosm_plan {
     long long prof_n[2] = {0, 0};
     std::vector<char> dm_set;
     int launch_mode = MSOSM_LAUNCH_PERSISTENT;
-    // accumulate launch (lengths 2^14..2^16, delays < 256): ring, H and dH also in its layout
-    // (perm_pos; the same arrays when R = 1) and the delays as bytes in thread order
+    // accumulate launch (lengths 2^14..2^16, delays < 256): the ring also in its layout (perm_pos; the
+    // same array when R = 1), H and dH in its chirp layout (acc_chirp_pos) and the delays as bytes in
+    // thread order
     bool accum_ok = false;
     int perm_rl = 0, perm_sl = 0;
     float2 *ring_p = nullptr, *H_p = nullptr, *Hdot_p = nullptr;
@@ -519,8 +520,8 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         if (e == hipSuccess) e = hipMemset(p->delay_acc, 0, D * L);
         if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->ring_p, (size_t)p->FBS * L * sizeof(float2));
         if (e == hipSuccess && p->perm_rl) e = hipMemset(p->ring_p, 0, (size_t)p->FBS * L * sizeof(float2));
-        if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->H_p, D * L * sizeof(float2));
-        if (e == hipSuccess && p->perm_rl) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
+        if (e == hipSuccess) e = dalloc(p, &p->H_p, D * L * sizeof(float2));
+        if (e == hipSuccess) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
     }
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_c64, hipEventDisableTiming);
@@ -551,8 +552,8 @@ extern "C" int msosm_plan_destroy(msosm_plan *p)
 }
 
 // copy L complex values to the device multiplied by 1/L (exact for normal values: a power of two), in
-// natural order and, for a plan with a separate accumulate layout, also in that layout (bin k at
-// perm_pos(k))
+// natural order and, for a plan with the accumulate launch, also in its chirp layout (bin k at
+// acc_chirp_pos(perm_pos(k)))
 static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src)
 {
     const int L = p->L;
@@ -562,7 +563,7 @@ static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p,
     hipError_t e = hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess || !dst_p) return e;
     std::vector<float2> perm((size_t)L);
-    for (int k = 0; k < L; ++k) perm[perm_pos((uint32_t)k, p->perm_rl, p->perm_sl)] = tmp[k];
+    for (int k = 0; k < L; ++k) perm[acc_chirp_pos(perm_pos((uint32_t)k, p->perm_rl, p->perm_sl))] = tmp[k];
     return hipMemcpy(dst_p, perm.data(), perm.size() * sizeof(float2), hipMemcpyHostToDevice);
 }
 
@@ -609,8 +610,8 @@ extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
         hipError_t e = dalloc(p, &p->Hdot, hb);
         if (e == hipSuccess) e = dalloc(p, &p->tan_y, yb);
         if (e == hipSuccess) e = hipMemset(p->Hdot, 0, hb);
-        if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->Hdot_p, hb);
-        if (e == hipSuccess && p->perm_rl) e = hipMemset(p->Hdot_p, 0, hb);
+        if (e == hipSuccess && p->accum_ok) e = dalloc(p, &p->Hdot_p, hb);
+        if (e == hipSuccess && p->accum_ok) e = hipMemset(p->Hdot_p, 0, hb);
         if (e != hipSuccess)
             return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
         p->tan_set.assign(p->D, 0);
@@ -698,7 +699,7 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     ia.tw_s = p->tw_s;
     ia.order = p->order;
     ia.ring_p = p->ring_p ? p->ring_p : p->ring;
-    ia.H_p = p->H_p ? p->H_p : p->H;
+    ia.H_p = p->H_p;
     ia.delay_acc = p->delay_acc;
     if (p->fold) {
         msosm_fold *f = p->fold;
@@ -857,7 +858,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
         if (run_inv(prim) != MSOSM_OK) return MSOSM_ERR_HIP;
         InvArgs tan = prim;
         tan.H = p->Hdot;
-        tan.H_p = p->Hdot_p ? p->Hdot_p : p->Hdot;
+        tan.H_p = p->Hdot_p;
         tan.out_c64 = nullptr;
         tan.tangent_ref = p->tan_y;
         tan.bins = p->bins;

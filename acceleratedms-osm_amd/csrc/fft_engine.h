@@ -221,6 +221,21 @@ __device__ __forceinline__ void butterflies(float2 (&v)[16], int jt, const float
     }
 }
 
+// Barrier of an LDS exchange. WAVE: every point a wave reads was written by the same wave (its
+// columns' regions are its own), and LDS instructions of one wave execute in order, so a
+// wavefront-scope fence that keeps the compiler from moving accesses across it suffices.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_barrier()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Stockham output position of element t of butterfly q after a pass (radix R, span Ns).
 template <int NS, int R, int Ns>
 __device__ __forceinline__ int out_pos(int jt, int q, int t)
@@ -241,7 +256,7 @@ __device__ __forceinline__ int in_pos(int jt, int q, int t)
 
 // Remaining passes after the one that ended at span Ns_done (radix R_done).
 // On return, v holds the final outputs; element (q, t) is output index out_pos<NS, RL, NsL>.
-template <int NS, int NCOL, bool COLFAST, int DIR, int R_done, int Ns_done>
+template <int NS, int NCOL, bool COLFAST, int DIR, int R_done, int Ns_done, bool WAVE = false>
 __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int col, int jt,
                                               const float2 *__restrict__ tw)
 {
@@ -249,7 +264,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
     if constexpr (Ns < NS) {
         constexpr int R = PassRadix<NS, Ns>::R;
         constexpr int TPC = NS / 16;
-        __syncthreads(); // LDS free (previous readers done)
+        exchange_barrier<WAVE>(); // LDS free (previous readers done)
         // output position (j - k) R + k + t Ns, j = jt + q TPC, k = j mod Ns: split into a per-thread
         // part and a compile-time part without carries when k does not depend on q
         constexpr bool out_split = (Ns_done == 1 && R_done == 16) || (Ns_done % 16 == 0 && TPC % Ns_done == 0);
@@ -269,7 +284,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
                 for (int t = 0; t < R_done; ++t)
                     lds[lds_index<NS, NCOL, COLFAST>(out_pos<NS, R_done, Ns_done>(jt, q, t), col)] = v[q * R_done + t];
         }
-        __syncthreads();
+        exchange_barrier<WAVE>();
         // input position jt + q TPC + t NS/R
         constexpr bool in_split = TPC % 16 == 0 && (NS / R) % 16 == 0;
         if constexpr (in_split) {
@@ -287,7 +302,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
                     v[q * R + t] = lds[lds_index<NS, NCOL, COLFAST>(in_pos<NS, R>(jt, q, t), col)];
         }
         butterflies<NS, R, Ns, DIR>(v, jt, tw);
-        stockham_rest<NS, NCOL, COLFAST, DIR, R, Ns>(v, lds, col, jt, tw);
+        stockham_rest<NS, NCOL, COLFAST, DIR, R, Ns, WAVE>(v, lds, col, jt, tw);
     }
 }
 
@@ -304,12 +319,13 @@ struct LastPass<NS, NS> {
 };
 
 // Full sub-FFT. On entry v[t] = x[jt + t * NS/16] (t in [0,16)). On exit element (q, t) of v is
-// output index final_pos<NS>(jt, q, t).
-template <int NS, int NCOL, bool COLFAST, int DIR>
+// output index final_pos<NS>(jt, q, t). WAVE: the NS/16 threads of every column are lanes of one
+// wave and the caller has made the columns' LDS regions free (exchange_barrier).
+template <int NS, int NCOL, bool COLFAST, int DIR, bool WAVE = false>
 __device__ __forceinline__ void subfft(float2 (&v)[16], float2 *lds, int col, int jt, const float2 *__restrict__ tw)
 {
     butterflies<NS, 16, 1, DIR>(v, jt, tw);
-    stockham_rest<NS, NCOL, COLFAST, DIR, 16, 1>(v, lds, col, jt, tw);
+    stockham_rest<NS, NCOL, COLFAST, DIR, 16, 1, WAVE>(v, lds, col, jt, tw);
 }
 
 template <int NS>

@@ -31,6 +31,18 @@ __host__ __device__ __forceinline__ uint32_t perm_pos(uint32_t k, int rl, int sl
     return ((k & ((1u << rl) - 1u)) << sl) | (k >> rl);
 }
 
+// Index of accumulate-layout position P = r S + p + 1024 t (S = 2^14; p in [0, 1024) the column index
+// of the transform's thread tid = 256 (p mod 128 div 32) + p mod 32 + 32 (p div 128)) in the accumulate
+// launch's chirp table: values t = 2i, 2i + 1 of thread tid form the 16-byte word of lane tid mod 64
+// in row i of its wave's block, i.e. index r S + 1024 (tid div 64) + 128 i + 2 (tid mod 64) + (t mod 2),
+// so that each of a wave's eight chirp loads reads 1 KiB in one run.
+__host__ __device__ __forceinline__ uint32_t acc_chirp_pos(uint32_t P)
+{
+    const uint32_t r = P >> 14, q = P & 16383u, p = q & 1023u, t = q >> 10;
+    const uint32_t tid = ((p & 127u) >> 5) * 256u + (p & 31u) + 32u * (p >> 7);
+    return (r << 14) + (tid >> 6) * 1024u + (t >> 1) * 128u + (tid & 63u) * 2u + (t & 1u);
+}
+
 struct InvArgs {
     const float2 *ring;
     int FBS;
@@ -51,10 +63,10 @@ struct InvArgs {
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
     int order;                // unit order (MSOSM_ORDER_*): which units run together
-    // accumulate launch (lengths 2^14..2^16): ring and H in the layout perm_pos with R = L / 2^14, and
-    // the delays as bytes in thread order: the 16 delays a thread gathers for transform r are the 16
-    // bytes at r 2^14 + 16 p (p = its column index in [0, 1024)), the t-th one being bin position
-    // r 2^14 + p + 1024 t
+    // accumulate launch (lengths 2^14..2^16): the ring in the layout perm_pos with R = L / 2^14, H with
+    // bin k at acc_chirp_pos(perm_pos(k)), and the delays as bytes in thread order: the 16 delays a
+    // thread gathers for transform r are the 16 bytes at r 2^14 + 16 p (p = its column index in
+    // [0, 1024)), the t-th one being bin position r 2^14 + p + 1024 t
     const float2 *ring_p, *H_p;
     const uint8_t *delay_acc;
 };

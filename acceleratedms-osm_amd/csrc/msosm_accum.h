@@ -12,9 +12,10 @@
 // with W = e^{+2 pi i / L} and j in [0, S). The four-step intermediate never leaves the CU: the launch
 // reads the gathered spectra (ring, chirp, delays) and writes only its outputs.
 //
-// The launch reads its own copies of the ring and chirp, with bin k at position P(k) = (k mod R) S + k div R
-// (perm_pos), so that the gather of C_r reads contiguous runs, and the delays as bytes in thread order
-// (one 16-byte load per thread and transform).
+// The launch reads its own copies of the ring, with bin k at position P(k) = (k mod R) S + k div R
+// (perm_pos), so that the gather of C_r reads contiguous runs, of the chirp, in the order its threads
+// use it (acc_chirp_pos: eight 1 KiB runs per wave and transform, loaded straight into LDS), and of the
+// delays as bytes in thread order (one 16-byte load per thread and transform).
 #pragma once
 #include "msosm_device.h"
 
@@ -76,6 +77,25 @@ __device__ __forceinline__ float2 acc_wl(const AccTw &tw, int m)
     return cmul(tw.hi[m >> tw.s], tw.lo[m & ((1 << tw.s) - 1)]);
 }
 
+// LDS rows of wave w in the pass-B exchange layout (8 rows of ACC_N + ACC_N / 16 words): where the
+// wave's chirp values are staged
+__device__ __forceinline__ float2 *acc_wave_rows(float2 *lds_all, int w)
+{
+    return lds_all + (w >> 2) * TileWords<ACC_N>::V + (w & 3) * 8 * (ACC_N + ACC_N / 16);
+}
+
+// Chirp values of transform r for wave w (acc_chirp_pos) from global memory straight into LDS: eight
+// 16-byte loads per lane, lane l of load i landing at rows + 128 i + 2 l; complete once the wave's
+// vector memory counter drains
+__device__ __forceinline__ void acc_chirp_to_lds(const float2 *Hr, float2 *rows, int w, int lane)
+{
+    const float2 *src = Hr + 1024 * w + 2 * lane;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + 128 * i),
+                                         (__attribute__((address_space(3))) void *)(rows + 128 * i), 16, 0, 0);
+}
+
 // The 16 byte delays of this thread for transform r of DM d: the bytes at r S + 16 p (p = k1 + 128 jtA),
 // the t-th one for bin position r S + p + 1024 t
 template <int R>
@@ -121,10 +141,13 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
         const int jtB = lt & 7, rowB = lt >> 3; // pass B: row n2 = 32 sub + rowB, columns jtB + 8 t
         const int k1 = sub * 32 + col, n2B = sub * 32 + rowB;
         float2 *tile = lds_all + sub * TileWords<ACC_N>::V;
-        // gather of C_r's inputs: bins R (k1 + 128 (jtA + 8 t)) + r at positions r S + k1 + 128 jtA + 1024 t
-        // gather: the ring values load straight into the transform's registers, the chirp follows in two
-        // halves of 8 values (register budget: the accumulators stay live), and the next transform's
-        // delays are requested once this one's ring loads are out
+        // gather of C_r's inputs: bins R (k1 + 128 (jtA + 8 t)) + r at positions r S + k1 + 128 jtA + 1024 t.
+        // The ring values load into the transform's registers and the chirp into the wave's LDS rows (free
+        // until pass A's first exchange, which starts with a workgroup barrier), both in one round trip;
+        // the chirp loads go out after the ring-offset lookups, which they would otherwise hold up (an LDS
+        // read after an LDS-bound load waits for it), and the next transform's delays after them
+        const int wv = tid >> 6, lane = tid & 63;
+        float2 *rows = acc_wave_rows(lds_all, wv);
         float2 v[16];
         {
             const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
@@ -135,24 +158,24 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                 const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p;
                 v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
             }
+            acc_chirp_to_lds(Hd + (size_t)r * S, rows, wv, lane);
             if (r + 1 < R)
                 dl = acc_delays<R>(a, d, r + 1);
             else if (next)
                 dl = acc_delays<R>(a, dn, 0);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
-                __builtin_amdgcn_sched_barrier(0);
-                float2 h[8];
+            for (int i = 0; i < 8; ++i) {
+                const float4 h = hrow[64 * i];
+                const float2 hv[2] = {make_float2(h.x, h.y), make_float2(h.z, h.w)};
 #pragma unroll
-                for (int t = 0; t < 8; ++t) h[t] = ld32(Hd, p0 + 1024u * (8 * hh + t));
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    float2 &y = v[8 * hh + t];
-                    y = cmul(y, h[t]);
+                for (int e = 0; e < 2; ++e) {
+                    float2 &y = v[2 * i + e];
+                    y = cmul(y, hv[e]);
                     if constexpr (f16) power += y.x * y.x + y.y * y.y;
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
         }
         // pass A: columns over k2, then W_L^{n2 (R k1 + r)} (the four-step twiddle times W_L^{r n2})
         subfft<ACC_N, 32, true, +1>(v, tile, col, jtA, tw.t128);
@@ -178,8 +201,10 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
 #pragma unroll
         for (int t = 0; t < 16; ++t) v[t] = lds_all[n2B * ACC_XS + jtB + 8 * t];
         __syncthreads(); // the transpose is read: the area becomes the tile exchanges again
-        // pass B: rows over k1; element (q, t) is n1 = jtB + 8 q + 16 t, j = n2B + 128 n1
-        subfft<ACC_N, 32, false, +1>(v, tile, rowB, jtB, tw.t128);
+        // pass B: rows over k1; element (q, t) is n1 = jtB + 8 q + 16 t, j = n2B + 128 n1. The 8 threads
+        // of a row are consecutive lanes and a row's exchange region (row-major, 136 words) is its own,
+        // so the exchanges need no workgroup barrier
+        subfft<ACC_N, 32, false, +1, true>(v, tile, rowB, jtB, tw.t128);
         // accumulate T_r[j] = W_L^{128 r n1} C'_r[j] (the factor W_L^{r n2} rode in pass A's twiddle):
         //   R = 2: acc0 += (-1)^r T;  R = 4: acc0 += (-1)^r T, acc1 += (-i)^r T;  R = 1: keep n1 >= 64.
         // One branch-free path for every r (r is wave-uniform): the factors come from the LDS table.
@@ -239,6 +264,14 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
     const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * M : nullptr;
     float *fb = reinterpret_cast<float *>(lds_all); // fold buffer: sample m at (m % MPT) FOLD_X + m / MPT
+    // the fold bins of the samples this thread folds, [MPT tid, MPT tid + MPT), requested before the
+    // outputs are written so that their latency overlaps that
+    uint4 bw[MPT / 8];
+    if (fold) {
+        const uint4 *bins4 = reinterpret_cast<const uint4 *>(a.bins + (size_t)b * M + (size_t)MPT * tid);
+#pragma unroll
+        for (int i = 0; i < MPT / 8; ++i) bw[i] = bins4[i];
+    }
     auto out1 = [&](int m, float2 y) {
         if (oc) st32(oc, (uint32_t)m, y);
         if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
@@ -270,12 +303,10 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     if (fold) {
         __syncthreads();
         // thread tid folds samples [MPT tid, MPT tid + MPT) with runs of equal bin merged in registers
-        const uint16_t *bins = a.bins + (size_t)b * M + (size_t)MPT * tid;
         uint16_t bv[MPT];
 #pragma unroll
         for (int i = 0; i < MPT / 8; ++i) {
-            const uint4 w4 = *reinterpret_cast<const uint4 *>(bins + 8 * i);
-            const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+            const uint32_t w[4] = {bw[i].x, bw[i].y, bw[i].z, bw[i].w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 bv[8 * i + 2 * k] = (uint16_t)(w[k] & 0xFFFFu);

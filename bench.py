@@ -234,7 +234,7 @@ def main():
     # the same design over the same pushes (identical counts; sums equal up to FP64 atomic order)
     check = timed_output_check(ms, design, data, pushes, args.warmup + args.steps, sums, cnts, order)
     log(f"timed-output check: {check}")
-    rel_err = {"dm_grid_int8": rel_err_dm_grid(ms, design, data)}
+    rel_err = {"dm_grid_int8": rel_err_dm_grid(ms, design, data, order)}
     log(f"rel err dm grid: {rel_err['dm_grid_int8']}")
 
     cpu = None
@@ -330,9 +330,10 @@ def rel_err_series(got, ref) -> tuple[float, float]:
     return l2, mx
 
 
-def rel_err_dm_grid(ms, design, data, ntrials: int = 4, calls: int = 2) -> dict:
+def rel_err_dm_grid(ms, design, data, order: int, ntrials: int = 4, calls: int = 2) -> dict:
     """Series of sampled trials of the grid (same M, design, int8 input) against the FP64 oracle over the
-    record's first calls pushes, from zero history in both."""
+    record's first calls pushes, from zero history in both, in the timed launch's form (accumulate,
+    role split or sequential: the unit order does not change the series bits)."""
     import numpy as np
 
     from oracle import oracle as orc
@@ -344,6 +345,7 @@ def rel_err_dm_grid(ms, design, data, ntrials: int = 4, calls: int = 2) -> dict:
     n = per * calls
     q = data[: 2 * n].cpu().numpy()
     plan = ms.Plan.from_design(sub, W["batch"], in_format=ms.IN_I8, out_flags=ms.OUT_C64, device=0)
+    plan.set_schedule(0, order)
     outs = []
     for c in range(calls):
         plan.push(q[2 * c * per: 2 * (c + 1) * per].reshape(-1, 2))
@@ -353,7 +355,8 @@ def rel_err_dm_grid(ms, design, data, ntrials: int = 4, calls: int = 2) -> dict:
     ref = orc.msosm(orc.int8_to_complex(q), design.M, design.delays[pick], design.H[pick].astype(np.complex128))
     errs = [rel_err_series(got[j], ref[j]) for j in range(ntrials)]
     return {"rel_l2_max": max(e[0] for e in errs), "max_abs_rel_max": max(e[1] for e in errs),
-            "trials": [float(d) for d in design.dms[pick]], "samples": n, "bar": 1e-5}
+            "trials": [float(d) for d in design.dms[pick]], "samples": n, "bar": 1e-5,
+            "schedule": order_name(order)}
 
 
 def bench_uneven(args, nested: bool = False):
