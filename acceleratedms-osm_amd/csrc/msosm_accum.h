@@ -142,11 +142,11 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
         const int k1 = sub * 32 + col, n2B = sub * 32 + rowB;
         float2 *tile = lds_all + sub * TileWords<ACC_N>::V;
         // gather of C_r's inputs: bins R (k1 + 128 (jtA + 8 t)) + r at positions r S + k1 + 128 jtA + 1024 t.
-        // The ring values load into the transform's registers and the chirp into the wave's LDS rows (free
-        // until pass A's first exchange, which starts with a workgroup barrier), both in one round trip;
-        // the chirp loads go out after the ring-offset lookups, which they would otherwise hold up (an LDS
-        // read after an LDS-bound load waits for it), and the next transform's delays after them
-        const int wv = tid >> 6, lane = tid & 63;
+        // The ring values load into the transform's registers; the chirp is in the wave's LDS rows, put
+        // there during the previous transform's combine or, for r = 0, requested here after the ring-offset
+        // lookups (which it would otherwise hold up: an LDS read after an LDS-bound load waits for it), in
+        // the same round trip as the ring. The next transform's delays follow.
+        const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63; // wv: wave-uniform (SGPRs)
         float2 *rows = acc_wave_rows(lds_all, wv);
         float2 v[16];
         {
@@ -158,7 +158,7 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                 const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p;
                 v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
             }
-            acc_chirp_to_lds(Hd + (size_t)r * S, rows, wv, lane);
+            if (r == 0) acc_chirp_to_lds(Hd, rows, wv, lane);
             if (r + 1 < R)
                 dl = acc_delays<R>(a, d, r + 1);
             else if (next)
@@ -223,6 +223,14 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
             // (chains of at most 8 products)
             const float2 cq = acc_wl(tw, 1024 * r), ct = acc_wl(tw, 2048 * r);
             float2 w = acc_wl(tw, 128 * r * jtB);
+            // the next transform's chirp into this wave's rows: their pass-B exchange reads are complete,
+            // no other wave touches them before the next pass A's first barrier, and the combine below reads
+            // no LDS. Without a branch on r (one costs the register allocator dearly) the last transform
+            // requests its own chirp again; that request drains before the rows become the fold buffer.
+            // The scheduling barrier keeps the requests ahead of the combine.
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            acc_chirp_to_lds(Hd + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
 #pragma unroll
@@ -239,8 +247,11 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                 if (t + 1 < 8) w = cmul(w, ct);
             }
         }
-        __syncthreads(); // the tile exchanges are done before the next gather's transform uses them
+        // no barrier here: the next transform's first cross-wave LDS write follows pass A's first
+        // workgroup barrier, which every wave reaches only after this combine and its next chirp reads
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); // the last chirp request has landed
+    __syncthreads(); // every wave's pass B is done: the area becomes the fold buffer
 
     // outputs: element e of acc0 is sample m = j (R = 2, 4) or j - S/2 (R = 1), acc1 is m = j + S
     int tid = threadIdx.x;
