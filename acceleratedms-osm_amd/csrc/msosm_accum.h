@@ -107,6 +107,12 @@ __device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int r)
     return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (R * ACC_S) + (size_t)r * ACC_S + 16u * (uint32_t)p);
 }
 
+// Transform index of acc_unit's body: V >= 0 known at compile time, V = -1 a run-time r
+template <int V>
+struct AccR {
+    static constexpr int value = V;
+};
+
 // One dedispersion unit in the accumulate form. hsum/hcnt: the LDS fold histogram (zero on entry and
 // exit); pw_lds: the FP16-scale word (zero on entry and exit).
 // dl holds this unit's first delay words on entry and, on return, those of the next unit (DM dn, block
@@ -125,13 +131,18 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring_p;
     float power = 0.f;
+    // R = 4: transform 0 has its own code (its combine factors are 1: the accumulators start as its
+    // outputs) and one loop runs r = 1..3; R = 1, 2: one loop from zeroed accumulators (a separate
+    // transform 0 costs registers there, and a fully unrolled R = 4 spills heavily)
+    constexpr bool peel0 = R == 4;
     float2 acc0[R == 1 ? 8 : 16], acc1[R == 4 ? 16 : 1];
+    if constexpr (!peel0) {
 #pragma unroll
-    for (int e = 0; e < (R == 1 ? 8 : 16); ++e) acc0[e] = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int e = 0; e < (R == 4 ? 16 : 1); ++e) acc1[e] = make_float2(0.f, 0.f);
-#pragma unroll 1
-    for (int r = 0; r < R; ++r) {
+        for (int e = 0; e < (R == 1 ? 8 : 16); ++e) acc0[e] = make_float2(0.f, 0.f);
+    }
+    // transform r: gather, C_r, and its share of the kept outputs (RC = r, or -1 for a run-time r)
+    auto transform = [&](auto rc_, int r) {
+        constexpr int RC = decltype(rc_)::value;
         // per-thread indices from an opaque thread index, so that their addresses are formed here and
         // not hoisted out of the loop (where they would stay live beside the accumulators)
         int tid = threadIdx.x;
@@ -158,7 +169,7 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                 const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p;
                 v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
             }
-            if (r == 0) acc_chirp_to_lds(Hd, rows, wv, lane);
+            if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Hd, rows, wv, lane);
             if (r + 1 < R)
                 dl = acc_delays<R>(a, d, r + 1);
             else if (next)
@@ -214,15 +225,14 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
 #pragma unroll
                 for (int t = 4; t < 8; ++t) acc0[q * 4 + t - 4] = v[q * 8 + t];
         } else {
-            // one path for every r (a branch on r here costs the register allocator dearly): at r = 0 the
-            // factors are 1
-            const float sg = (r & 1) ? -1.f : 1.f;                 // (-1)^r
-            const float cr = (r & 1) ? 0.f : (r & 2) ? -1.f : 1.f; // (-i)^r = cr + i sr
-            const float sr = (r & 1) ? ((r & 2) ? 1.f : -1.f) : 0.f;
             // W_L^{128 r n1} for n1 = jtB + 8 q + 16 t: a per-thread start and two wave-uniform steps
-            // (chains of at most 8 products)
-            const float2 cq = acc_wl(tw, 1024 * r), ct = acc_wl(tw, 2048 * r);
-            float2 w = acc_wl(tw, 128 * r * jtB);
+            // (chains of at most 8 products); all 1 at r = 0
+            float2 cq = make_float2(1.f, 0.f), ct = cq, w = cq;
+            if constexpr (RC != 0) {
+                cq = acc_wl(tw, 1024 * r);
+                ct = acc_wl(tw, 2048 * r);
+                w = acc_wl(tw, 128 * r * jtB);
+            }
             // the next transform's chirp into this wave's rows: their pass-B exchange reads are complete,
             // no other wave touches them before the next pass A's first barrier, and the combine below reads
             // no LDS. Without a branch on r (one costs the register allocator dearly) the last transform
@@ -236,20 +246,57 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     const int e = q * 8 + t;
-                    const float2 y = cmul(v[e], q ? cmul(w, cq) : w);
-                    acc0[e].x += sg * y.x;
-                    acc0[e].y += sg * y.y;
-                    if constexpr (R == 4) {
-                        acc1[e].x += cr * y.x - sr * y.y;
-                        acc1[e].y += cr * y.y + sr * y.x;
+                    if constexpr (RC == 0) {
+                        // T_0 = C'_0: the accumulators start here
+                        acc0[e] = v[e];
+                        if constexpr (R == 4) acc1[e] = v[e];
+                    } else if constexpr (RC > 0) {
+                        // (-1)^r and (-i)^r known: signs and a swap
+                        const float2 y = cmul(v[e], q ? cmul(w, cq) : w);
+                        if constexpr (RC & 1) {
+                            acc0[e].x -= y.x;
+                            acc0[e].y -= y.y;
+                        } else {
+                            acc0[e].x += y.x;
+                            acc0[e].y += y.y;
+                        }
+                        if constexpr (R == 4) {
+                            if constexpr (RC == 1) { // -i y
+                                acc1[e].x += y.y;
+                                acc1[e].y -= y.x;
+                            } else if constexpr (RC == 2) {
+                                acc1[e].x -= y.x;
+                                acc1[e].y -= y.y;
+                            } else { // i y
+                                acc1[e].x -= y.y;
+                                acc1[e].y += y.x;
+                            }
+                        }
+                    } else {
+                        // one path for every run-time r (a branch on r here costs the register allocator
+                        // dearly): (-1)^r and (-i)^r = cr + i sr as factors
+                        const float sg = (r & 1) ? -1.f : 1.f;
+                        const float cr = (r & 1) ? 0.f : (r & 2) ? -1.f : 1.f;
+                        const float sr = (r & 1) ? ((r & 2) ? 1.f : -1.f) : 0.f;
+                        const float2 y = cmul(v[e], q ? cmul(w, cq) : w);
+                        acc0[e].x += sg * y.x;
+                        acc0[e].y += sg * y.y;
+                        if constexpr (R == 4) {
+                            acc1[e].x += cr * y.x - sr * y.y;
+                            acc1[e].y += cr * y.y + sr * y.x;
+                        }
                     }
                 }
-                if (t + 1 < 8) w = cmul(w, ct);
+                if constexpr (RC != 0)
+                    if (t + 1 < 8) w = cmul(w, ct);
             }
         }
         // no barrier here: the next transform's first cross-wave LDS write follows pass A's first
         // workgroup barrier, which every wave reaches only after this combine and its next chirp reads
-    }
+    };
+    if constexpr (peel0) transform(AccR<0>{}, 0);
+#pragma unroll 1
+    for (int r = peel0 ? 1 : 0; r < R; ++r) transform(AccR<-1>{}, r);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); // the last chirp request has landed
     __syncthreads(); // every wave's pass B is done: the area becomes the fold buffer
 
