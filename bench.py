@@ -439,7 +439,9 @@ def bench_uneven(args, nested: bool = False):
             "avg_launch_ms": avg_s * 1e3, "launches": one["launches"], "launches_per_push": one["launches"] / one["pushes"],
             "alg_bytes_per_launch": alg, "alg_bytes_per_dm_sample": per_sample}
     if not (args.graph and not nested) and mode == ms.LAUNCH_PERSISTENT:
-        roof.update(read_traffic("inv_multi_kernel", "msosm_uneven_c64", "one launch"))
+        roof.update(read_traffic("inv_multi_kernel", "msosm_uneven_c64", "one launch per push"))
+    elif mode == ms.LAUNCH_PER_CLASS:
+        roof.update(read_traffic("per_class", "msosm_uneven_c64", "one launch per class"))
     del data
     torch.cuda.empty_cache()
     plain = {"harness_batch_32": {"value": per_dm[32]["rate"], "pushes": per_dm[32]["pushes"],

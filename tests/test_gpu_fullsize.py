@@ -179,3 +179,63 @@ def test_uneven_grid_full_record():
         rel = scaled_rel(sums[0][d], cs[0])
         print(f"  DM {design.dms[d]:.1f} (L = {2 * M}): max bin rel diff vs CPU port {rel:.2e}", flush=True)
         assert rel < 1e-5
+
+
+def test_dm_grid_accumulate_full_record_matches_other_orders_and_two_pass():
+    # The launch the benchmark times (the accumulate form in 16-DM XCD tiles on every CU), at its shape:
+    # 512 trials over 0-100 pc/cm^3 at L = 2^16, 32 blocks per push over the whole 2^27-sample int8 record.
+    # Against the accumulate form in the linear unit order on half the CUs, every push's complex64 series
+    # (all 512 x 2^20 values, compared on the device) is bit-identical and the fused folds have identical
+    # counts and sums equal up to the order of their FP64 atomics. Against the sequential two-pass launch
+    # (another factorisation of the same transform) each push's series agrees to FP32 rounding and the
+    # fold counts are identical for every trial but DM 0 (an identity delay whose exact-zero input pairs
+    # sit at the I == 0 skip threshold).
+    from tests.util import device_view
+
+    fs, f0, nbins, period, batch, n = 64e6, 1218e6, 1024, 5.0e-3, 32, 1 << 27
+    design = ms.Design(fs, f0, np.linspace(0.0, 100.0, 512).astype(np.float32))
+    assert design.M == 1 << 15 and design.delaycount == 54
+    per = batch * design.M
+    g = torch.Generator(device="cuda:0").manual_seed(4321)
+    q_dev = torch.randint(-127, 128, (2 * n,), dtype=torch.int8, device="cuda:0", generator=g)
+    plans, folds, words, values = [], [], [], []
+    schedules = [(0, ms.ORDER_XCD_TILE16 | ms.SCHEDULE_ACCUM), (128, ms.ORDER_LINEAR | ms.SCHEDULE_ACCUM),
+                 (0, ms.ORDER_XCD_TILE16)]
+    for wg, order in schedules:
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_C64, device=0)
+        fold = ms.Fold(period, fs, per, nbins, num_series=512, device=0)
+        plan.attach_fold(fold)
+        plan.set_schedule(wg, order)
+        plans.append(plan)
+        folds.append(fold)
+        words.append(device_view(plan.device_output_c64(), 512 * per * 2, "<u4"))
+        values.append(device_view(plan.device_output_c64(), 512 * per * 2, "<f4"))
+    wg, order = plans[0].schedule()
+    assert wg == plans[0].info().workgroups and order & ms.SCHEDULE_ACCUM
+    print(f"accumulate full record: M={design.M} dc={design.delaycount} workgroups={wg} pushes={n // per}",
+          flush=True)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for c in range(n // per):
+        for plan in plans:
+            plan.push(q_dev.data_ptr() + c * 2 * per)
+        for plan in plans:
+            plan.synchronize()
+        if not torch.equal(words[0], words[1]):
+            diff = int((words[0] != words[1]).sum())
+            raise AssertionError(f"push {c}: {diff} series words differ between unit orders of the accumulate form")
+        a, b = values[0].view(512, -1), values[2].view(512, -1)
+        rel = float(((a - b).norm(dim=1) / b.norm(dim=1)).max())
+        worst = max(worst, rel)
+        assert rel < 2e-6, f"push {c}: accumulate vs two-pass rel-L2 {rel:.3e}"
+    print(f"  accumulate vs two-pass: worst per-trial rel-L2 over all pushes {worst:.2e}", flush=True)
+    _, s0, c0 = folds[0].read()
+    _, s1, c1 = folds[1].read()
+    _, s2, c2 = folds[2].read()
+    assert np.array_equal(c0, c1)
+    np.testing.assert_allclose(s0, s1, rtol=1e-12)
+    assert np.array_equal(c0[1:], c2[1:])
+    assert scaled_rel(s0[1:], s2[1:]) < 1e-5
+    for p_, f_ in zip(plans, folds):
+        p_.close()
+        f_.close()
