@@ -354,7 +354,10 @@ struct msosm_plan {
     // same array when R = 1), H and dH in its chirp layout (acc_chirp_pos) and the delays as bytes in
     // thread order
     bool accum_ok = false;
-    int perm_rl = 0, perm_sl = 0;
+    int perm_rl = 0, perm_sl = 0, perm_ql = 0;
+    // lengths 2^17, 2^18: the accumulate launch runs acc_q sub-units of 2^15 points per unit, with the
+    // sub-unit length's twiddle tables at these offsets of tw (acc_q = 1: one unit)
+    int acc_q = 1, tw_sub_lo = 0, tw_sub_hi = 0, tw_sub_128 = 0, tw_sub_s = 0;
     float2 *ring_p = nullptr, *H_p = nullptr, *Hdot_p = nullptr;
     uint8_t *delay_acc = nullptr;
     int sched_wg = 0, order = MSOSM_ORDER_XCD_DM_MAJOR; // tile schedule (0 workgroups = G)
@@ -452,11 +455,16 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         return fail(MSOSM_ERR_UNSUPPORTED, "plan_create: kernels need " + std::to_string(tiles_lds) +
                                                " B of LDS per workgroup, device allows " + std::to_string(max_lds));
     // the accumulate form stores the delays as bytes: blocks 0..255
-    p->accum_ok = accum_supported(log2L) && p->dc <= 256 && (unsigned long long)p->FBS * p->L < (1ull << 32);
+    // (the sub-unit form of 2^17 and 2^18 has no FP16-pair epilogue: its per-block scale spans sub-units)
+    const int acc_q = accum_subunits(log2L);
+    p->accum_ok = accum_supported(log2L) && p->dc <= 256 && (unsigned long long)p->FBS * p->L < (1ull << 32) &&
+                  (acc_q == 1 || !(p->out_flags & MSOSM_OUT_F16));
     const int acc_lds = p->accum_ok ? accum_lds_bytes(log2L) : 0;
     if (p->accum_ok) {
-        p->perm_rl = log2L - 14;
+        p->acc_q = acc_q;
+        p->perm_rl = acc_q > 1 ? 1 : log2L - 14;
         p->perm_sl = 14;
+        p->perm_ql = acc_q > 1 ? 15 : log2L;
     }
     p->max_fold_bins = std::min(2048, (max_lds - std::max(tiles_lds, acc_lds) - INV_EXTRA_LDS) / 12);
     e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12 + INV_EXTRA_LDS);
@@ -485,7 +493,15 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     p->tw_n2 = n1;
     p->tw_lo = n1 + n2;
     p->tw_hi = n1 + n2 + nlo;
-    const int ntw = n1 + n2 + nlo + nhi;
+    int ntw = n1 + n2 + nlo + nhi;
+    if (p->acc_q > 1) {
+        // the sub-unit length 2^15: lo (256), hi (128) split at 8 bits, and the 128-point roots
+        p->tw_sub_s = 8;
+        p->tw_sub_lo = ntw;
+        p->tw_sub_hi = ntw + 256;
+        p->tw_sub_128 = ntw + 256 + 128;
+        ntw += 256 + 128 + 128;
+    }
     std::vector<float2> tw(ntw);
     auto put = [&](int off, long k, long n) {
         double re, im;
@@ -499,6 +515,15 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         double re, im;
         unit_root((long)k << p->tw_s, (long)L, &re, &im);
         tw[p->tw_hi + k] = make_float2((float)re, (float)im);
+    }
+    if (p->acc_q > 1) {
+        for (int k = 0; k < 256; ++k) put(p->tw_sub_lo, k, 1l << 15);
+        for (int k = 0; k < 128; ++k) {
+            double re, im;
+            unit_root((long)k << 8, 1l << 15, &re, &im);
+            tw[p->tw_sub_hi + k] = make_float2((float)re, (float)im);
+        }
+        for (int k = 0; k < 128; ++k) put(p->tw_sub_128, k, 128);
     }
 
     e = dalloc(p, &p->tw, ntw * sizeof(float2));
@@ -520,8 +545,9 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         if (e == hipSuccess) e = hipMemset(p->delay_acc, 0, D * L);
         if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->ring_p, (size_t)p->FBS * L * sizeof(float2));
         if (e == hipSuccess && p->perm_rl) e = hipMemset(p->ring_p, 0, (size_t)p->FBS * L * sizeof(float2));
-        if (e == hipSuccess) e = dalloc(p, &p->H_p, D * L * sizeof(float2));
-        if (e == hipSuccess) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
+        // per DM: one chirp table, or one per sub-unit
+        if (e == hipSuccess) e = dalloc(p, &p->H_p, D * p->acc_q * L * sizeof(float2));
+        if (e == hipSuccess) e = hipMemset(p->H_p, 0, D * p->acc_q * L * sizeof(float2));
     }
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_c64, hipEventDisableTiming);
@@ -553,8 +579,9 @@ extern "C" int msosm_plan_destroy(msosm_plan *p)
 
 // copy L complex values to the device multiplied by 1/L (exact for normal values: a power of two), in
 // natural order and, for a plan with the accumulate launch, also in its chirp layout (bin k at
-// acc_chirp_pos(perm_pos(k)))
-static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src)
+// acc_chirp_pos(perm_pos(k))), or as the sub-unit chirp tables (launch_sub_chirp, on the plan's stream)
+// (dst_p holds q tables of L values: q = p->acc_q for H, 1 for dH/dDM, which has no sub-unit tables)
+static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src, int q)
 {
     const int L = p->L;
     std::vector<float2> tmp((size_t)L);
@@ -562,6 +589,10 @@ static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p,
     for (int k = 0; k < L; ++k) tmp[k] = make_float2(src[2 * k] * s, src[2 * k + 1] * s);
     hipError_t e = hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess || !dst_p) return e;
+    if (q > 1) {
+        e = launch_sub_chirp(dst, dst_p, p->log2L, q, p->stream);
+        return e == hipSuccess ? hipStreamSynchronize(p->stream) : e;
+    }
     std::vector<float2> perm((size_t)L);
     for (int k = 0; k < L; ++k) perm[acc_chirp_pos(perm_pos((uint32_t)k, p->perm_rl, p->perm_sl))] = tmp[k];
     return hipMemcpy(dst_p, perm.data(), perm.size() * sizeof(float2), hipMemcpyHostToDevice);
@@ -583,15 +614,18 @@ extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, 
     HIPCHK(hipStreamSynchronize(p->stream));
     HIPCHK(hipMemcpy(p->delay + (size_t)i * p->L, dl.data(), p->L * sizeof(uint16_t), hipMemcpyHostToDevice));
     // the device copy carries the inverse transform's 1/L (exact: a power of two)
-    HIPCHK(upload_scaled(p, p->H + (size_t)i * p->L, p->H_p ? p->H_p + (size_t)i * p->L : nullptr, H));
+    HIPCHK(upload_scaled(p, p->H + (size_t)i * p->L, p->H_p ? p->H_p + (size_t)i * p->acc_q * p->L : nullptr, H,
+                         p->acc_q));
     if (p->accum_ok) {
-        // byte delays in thread order: position r S + p + 1024 t (perm_pos) goes to byte r S + 16 p + t
+        // byte delays in thread order: position r S + p + 1024 t of a run (perm_pos_sub; one run of L bins,
+        // or one per sub-unit spectrum) goes to byte r S + 16 p + t of that run
         std::vector<uint8_t> db((size_t)p->L);
         const uint32_t S = 1u << 14;
         for (int k = 0; k < p->L; ++k) {
-            const uint32_t pos = perm_pos((uint32_t)k, p->perm_rl, p->perm_sl);
-            const uint32_t r = pos / S, q = pos % S;
-            db[(size_t)r * S + 16 * (q & 1023u) + (q >> 10)] = (uint8_t)delay_points[k];
+            const uint32_t pos = perm_pos_sub((uint32_t)k, p->perm_rl, p->perm_sl, p->perm_ql);
+            const uint32_t run = pos >> p->perm_ql << p->perm_ql, w = pos - run;
+            const uint32_t r = w / S, q = w % S;
+            db[(size_t)run + (size_t)r * S + 16 * (q & 1023u) + (q >> 10)] = (uint8_t)delay_points[k];
         }
         HIPCHK(hipMemcpy(p->delay_acc + (size_t)i * p->L, db.data(), db.size(), hipMemcpyHostToDevice));
     }
@@ -610,13 +644,16 @@ extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
         hipError_t e = dalloc(p, &p->Hdot, hb);
         if (e == hipSuccess) e = dalloc(p, &p->tan_y, yb);
         if (e == hipSuccess) e = hipMemset(p->Hdot, 0, hb);
-        if (e == hipSuccess && p->accum_ok) e = dalloc(p, &p->Hdot_p, hb);
-        if (e == hipSuccess && p->accum_ok) e = hipMemset(p->Hdot_p, 0, hb);
+        // the accumulate layout of dH/dDM where a tangent push can run the accumulate form (not the
+        // sub-unit form of 2^17 and 2^18, whose tangent pushes run the two-pass form)
+        const bool acc_tan = p->accum_ok && p->acc_q == 1;
+        if (e == hipSuccess && acc_tan) e = dalloc(p, &p->Hdot_p, hb);
+        if (e == hipSuccess && acc_tan) e = hipMemset(p->Hdot_p, 0, hb);
         if (e != hipSuccess)
             return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
         p->tan_set.assign(p->D, 0);
     }
-    HIPCHK(upload_scaled(p, p->Hdot + (size_t)i * p->L, p->Hdot_p ? p->Hdot_p + (size_t)i * p->L : nullptr, dH));
+    HIPCHK(upload_scaled(p, p->Hdot + (size_t)i * p->L, p->Hdot_p ? p->Hdot_p + (size_t)i * p->L : nullptr, dH, 1));
     p->tan_set[i] = 1;
     return MSOSM_OK;
 }
@@ -654,7 +691,7 @@ static int collect_profile(msosm_plan *p)
 // Arguments of every launch of one push (no launch): forward transforms of this push's blocks into
 // the ring, the overlap history for the next push, the fold bins (if a fold is attached) and the
 // dedispersion with the attached fold.
-void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
+void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out, bool allow_sub)
 {
     PushArgs &a = *out;
     a = PushArgs{};
@@ -676,6 +713,7 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     fa.ring_p = p->ring_p; // a second copy only where the accumulate layout differs
     fa.perm_rl = p->perm_rl;
     fa.perm_sl = p->perm_sl;
+    fa.perm_ql = p->perm_ql;
     a.log2L = p->log2L;
     a.fwd_grid = std::min(p->B, p->G);
     a.hist = HistArgs{p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M};
@@ -701,10 +739,21 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
     ia.ring_p = p->ring_p ? p->ring_p : p->ring;
     ia.H_p = p->H_p;
     ia.delay_acc = p->delay_acc;
+    // the sub-unit accumulate launch where the schedule asks for it and the caller runs it (not for the
+    // one-launch uneven kernel or a tangent push): its tables, and its order of the fold bins
+    const bool sub = allow_sub && p->acc_q > 1 && (p->order & MSOSM_SCHEDULE_ACCUM);
+    if (p->acc_q > 1 && !sub) ia.order &= ~MSOSM_SCHEDULE_ACCUM;
+    if (sub) {
+        ia.sub_lo = tw + p->tw_sub_lo;
+        ia.sub_hi = tw + p->tw_sub_hi;
+        ia.sub_128 = tw + p->tw_sub_128;
+        ia.sub_s = p->tw_sub_s;
+    }
     if (p->fold) {
         msosm_fold *f = p->fold;
         a.has_bins = true;
-        a.bins = BinsArgs{p->bins, (long long)p->B * p->M, f->nbins, f->t_start, f->dt, f->period};
+        a.bins = BinsArgs{p->bins, (long long)p->B * p->M, f->nbins, f->t_start, f->dt, f->period, sub ? p->acc_q : 1,
+                          p->M};
         ia.bins = p->bins;
         ia.fold_sum = f->sums;
         ia.fold_cnt = f->cnts;
@@ -714,10 +763,10 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out)
 
 // The part of a push before the dedispersion launch: launches the forward transforms, the history
 // copy and the fold bins, and returns the dedispersion arguments (the launch is the caller's).
-int msosm::plan_push_prepare(msosm_plan *p, const void *src, InvArgs *out)
+int msosm::plan_push_prepare(msosm_plan *p, const void *src, InvArgs *out, bool allow_sub)
 {
     PushArgs a;
-    plan_push_args(p, src, &a);
+    plan_push_args(p, src, &a, allow_sub);
     EventPair ev{};
     if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
     HIPCHK(launch_fwd(a.log2L, a.fwd, a.fwd_grid, p->stream));
@@ -726,7 +775,10 @@ int msosm::plan_push_prepare(msosm_plan *p, const void *src, InvArgs *out)
         p->pending[0].push_back(ev);
     }
     HIPCHK(launch_hist(a.hist.hist, a.hist.in, a.hist.fmt, a.hist.first, a.hist.M, p->stream));
-    if (a.has_bins)
+    if (a.has_bins && a.bins.q > 1)
+        HIPCHK(launch_bins_sub(a.bins.bins, a.bins.n, a.bins.nbins, a.bins.t_start, a.bins.dt, a.bins.period, a.bins.q,
+                               a.bins.m, p->stream));
+    else if (a.has_bins)
         HIPCHK(launch_bins(a.bins.bins, a.bins.n, a.bins.nbins, a.bins.t_start, a.bins.dt, a.bins.period, p->stream));
     *out = a.inv;
     return MSOSM_OK;
@@ -812,7 +864,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
     if (p->out_c64) HIPCHK(hipStreamWaitEvent(p->stream, p->d2h_c64, 0));
     if (p->out_f16) HIPCHK(hipStreamWaitEvent(p->stream, p->d2h_f16, 0));
     InvArgs ia{};
-    if (plan_push_prepare(p, src, &ia) != MSOSM_OK) return MSOSM_ERR_HIP;
+    if (plan_push_prepare(p, src, &ia, tfold == nullptr) != MSOSM_OK) return MSOSM_ERR_HIP;
     if (mode != MSOSM_INPUT_DEVICE) HIPCHK(hipEventRecord(p->in_ev[slot], p->stream));
     if (tfold && !p->fold)
         HIPCHK(launch_bins(p->bins, (long long)nsamp, tfold->nbins, tfold->t_start, tfold->dt, tfold->period,
@@ -827,7 +879,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
                 one.D = 1;
                 one.H = x.H + (size_t)d * p->L;
                 one.delay = x.delay + (size_t)d * p->L;
-                one.H_p = x.H_p + (size_t)d * p->L;
+                one.H_p = x.H_p + (size_t)d * p->acc_q * p->L;
                 if (x.delay_acc) one.delay_acc = x.delay_acc + (size_t)d * p->L;
                 if (x.out_c64) one.out_c64 = x.out_c64 + d * ser;
                 if (x.out_u16) one.out_u16 = x.out_u16 + d * ser;

@@ -292,7 +292,7 @@ static int graph_push(msosm_multi *m, const void *src)
     m->gargs.resize(n);
     std::vector<InvArgs> inv(n);
     for (int i = 0; i < n; ++i) {
-        plan_push_args(m->classes[i].plan, src, &m->gargs[i]);
+        plan_push_args(m->classes[i].plan, src, &m->gargs[i], false);
         inv[i] = m->gargs[i].inv;
     }
     multi_inv_args(m, m->gma, inv);
@@ -377,7 +377,7 @@ static int fused_push(msosm_multi *m, const void *src)
     const int n = (int)m->classes.size();
     std::vector<InvArgs> inv(n);
     for (int i = 0; i < n; ++i) {
-        const int rc = plan_push_prepare(m->classes[i].plan, src, &inv[i]);
+        const int rc = plan_push_prepare(m->classes[i].plan, src, &inv[i], false);
         if (rc != MSOSM_OK) return rc;
     }
     MultiInvArgs ma;

@@ -1,6 +1,7 @@
-// Persistent dedispersion kernel of the accumulate form (FFT lengths 2^14, 2^15, 2^16): one workgroup
-// per CU walks (DM, block) units in the plan's unit order; each unit's inverse transform stays in the
-// CU (LDS and registers), so the launch reads the gathered spectra and writes only its outputs.
+// Persistent dedispersion kernels of the accumulate form: one workgroup per CU walks (DM, block) units
+// in the plan's unit order (FFT lengths 2^14, 2^15, 2^16), or (DM, block, sub-unit) items (lengths 2^17,
+// 2^18: Q = L / 2^15 sub-units of 2^15 points); each unit's inverse transform stays in the CU (LDS and
+// registers), so the launch reads the gathered spectra and writes only its outputs.
 #include "msosm_accum.h"
 
 namespace msosm {
@@ -23,13 +24,13 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
     // the workgroup's units in the plan's order, each unit's first delays and ring offsets prepared
     // during the previous one
-    uint4 dl = have ? acc_delays<R>(a, d, 0) : make_uint4(0u, 0u, 0u, 0u);
+    uint4 dl = have ? acc_delays<R>(a, d, 0, 0) : make_uint4(0u, 0u, 0u, 0u);
     int cur = 0;
 #pragma unroll 1
     while (have) {
         int dn = 0, bn = 0;
         const bool next = o.next(s, a, dn, bn);
-        acc_unit<R, RING32, VAR>(a, d, b, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, slots + 256 * cur,
+        acc_unit<R, RING32, VAR>(a, d, b, 0, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, slots + 256 * cur,
                                  slots + 256 * (cur ^ 1));
         cur ^= 1;
         d = dn;
@@ -38,12 +39,63 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     }
 }
 
-static int acc_r(int log2L) { return log2L == 14 ? 1 : log2L == 15 ? 2 : log2L == 16 ? 4 : 0; }
+// Sub-unit accumulate kernel (L = Q 2^15, plain epilogue: complex64 / uint16 outputs, fused fold). Items
+// x = (d B + b) Q + c in the XCD-grouped order: the G/8 workgroups of an XCD take consecutive items, so
+// they run the Q sub-units of a block and the blocks of one DM together, sharing the blocks' ring values
+// and the DM's chirp tables in their L2.
+template <int Q, bool RING32>
+__global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
+{
+    constexpr int R = 2;
+    using A = AccShape<R>;
+    extern __shared__ float2 lds_all[];
+    uint32_t *slots = reinterpret_cast<uint32_t *>(lds_all + ACC_AREA + A::TW_WORDS); // two tables of 256
+    const long long total = (long long)a.D * a.B * Q;
+    const int G = gridDim.x, w = blockIdx.x;
+    long long x = G % 8 == 0 ? (long long)(w % 8) * (G / 8) + w / 8 : w;
+    auto item = [&](long long v, int &d, int &b, int &c) {
+        c = (int)(v % Q);
+        const long long u = v / Q;
+        b = (int)(u % a.B);
+        d = (int)(u / a.B);
+    };
+    int d = 0, b = 0, c = 0;
+    bool have = x < total; // uniform over the workgroup
+    if (have) item(x, d, b, c);
+    if (have) acc_slot_table<R, Q>(a, b, slots); // made visible by the twiddle load's barrier
+    const AccTw tw = acc_load_twiddles<R, Q>(lds_all, a);
+    double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + A::LDS_BYTES);
+    unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
+    float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
+    fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
+    uint4 dl = have ? acc_delays<R, Q>(a, d, 0, 0) : make_uint4(0u, 0u, 0u, 0u);
+    int cur = 0;
+#pragma unroll 1
+    while (have) {
+        const long long xn = x + G;
+        const bool next = xn < total;
+        int dn = 0, bn = 0, cn = 0;
+        if (next) item(xn, dn, bn, cn);
+        acc_unit<R, RING32, VAR_PLAIN, Q>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn,
+                                          slots + 256 * cur, slots + 256 * (cur ^ 1));
+        cur ^= 1;
+        x = xn;
+        d = dn;
+        b = bn;
+        c = cn;
+        have = next;
+    }
+}
 
-bool accum_supported(int log2L) { return acc_r(log2L) != 0; }
+static int acc_r(int log2L) { return log2L == 14 ? 1 : log2L == 15 ? 2 : log2L == 16 ? 4 : log2L <= 18 ? 2 : 0; }
+
+int accum_subunits(int log2L) { return log2L == 17 ? 4 : log2L == 18 ? 8 : 1; }
+
+bool accum_supported(int log2L) { return log2L >= 14 && log2L <= 18; }
 
 int accum_lds_bytes(int log2L)
 {
+    if (!accum_supported(log2L)) return -1;
     switch (acc_r(log2L)) {
     case 1: return AccShape<1>::LDS_BYTES;
     case 2: return AccShape<2>::LDS_BYTES;
@@ -68,12 +120,26 @@ static hipError_t configure_acc_t(int bytes)
     return hipSuccess;
 }
 
+template <int Q>
+static hipError_t configure_acc_sub_t(int bytes)
+{
+    const void *fs[] = {reinterpret_cast<const void *>(&acc_sub_kernel<Q, true>),
+                        reinterpret_cast<const void *>(&acc_sub_kernel<Q, false>)};
+    for (const void *f : fs) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t configure_accum_lds(int log2L, int bytes)
 {
-    switch (acc_r(log2L)) {
-    case 1: return configure_acc_t<1>(bytes);
-    case 2: return configure_acc_t<2>(bytes);
-    case 4: return configure_acc_t<4>(bytes);
+    if (log2L == 17) return configure_acc_sub_t<4>(bytes);
+    if (log2L == 18) return configure_acc_sub_t<8>(bytes);
+    switch (log2L) {
+    case 14: return configure_acc_t<1>(bytes);
+    case 15: return configure_acc_t<2>(bytes);
+    case 16: return configure_acc_t<4>(bytes);
     }
     return hipErrorInvalidValue;
 }
@@ -99,12 +165,29 @@ static hipError_t launch_acc_t(const InvArgs &a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int Q>
+static hipError_t launch_acc_sub_t(const InvArgs &a, int grid, hipStream_t st)
+{
+    using A = AccShape<2>;
+    // the sub-unit form has the plain epilogue only; the caller chooses another form otherwise
+    if (a.tangent_ref || a.out_f16 || !a.sub_lo) return hipErrorInvalidValue;
+    const size_t lds = A::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0) + INV_EXTRA_LDS;
+    const bool ring32 = (unsigned long long)a.FBS * (A::L * Q) * sizeof(float2) < (1ull << 32);
+    if (ring32)
+        hipLaunchKernelGGL((acc_sub_kernel<Q, true>), dim3(grid), dim3(WG), lds, st, a);
+    else
+        hipLaunchKernelGGL((acc_sub_kernel<Q, false>), dim3(grid), dim3(WG), lds, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_accum(int log2L, const InvArgs &a, int grid, hipStream_t st)
 {
-    switch (acc_r(log2L)) {
-    case 1: return launch_acc_t<1>(a, grid, st);
-    case 2: return launch_acc_t<2>(a, grid, st);
-    case 4: return launch_acc_t<4>(a, grid, st);
+    switch (log2L) {
+    case 14: return launch_acc_t<1>(a, grid, st);
+    case 15: return launch_acc_t<2>(a, grid, st);
+    case 16: return launch_acc_t<4>(a, grid, st);
+    case 17: return launch_acc_sub_t<4>(a, grid, st);
+    case 18: return launch_acc_sub_t<8>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

@@ -43,32 +43,35 @@ struct AccShape {
     static_assert(MPT * FOLD_X <= ACC_AREA * 2, "fold buffer fits");
 };
 
-// the tables of the plan (one-length layout: n1, n2, lo, hi with s = (log2 L + 1) / 2) into LDS
-template <int R>
+// the tables of the plan (one-length layout: n1, n2, lo, hi with s = (log2 L + 1) / 2) into LDS; with
+// Q > 1 sub-units, the plan's tables of the sub-unit length
+template <int R, int Q = 1>
 __device__ __forceinline__ AccTw acc_load_twiddles(float2 *lds_all, const InvArgs &a)
 {
     using A = AccShape<R>;
     float2 *t = lds_all + ACC_AREA;
     float2 *lo = t + 128, *hi = lo + 256;
     // a 128-point table: the plan's N2 table has 128 entries for L <= 2^15, its N1 table 256 at 2^16
-    const float2 *src128 = R == 4 ? a.tw_n1 : a.tw_n2;
+    const float2 *src128 = Q > 1 ? a.sub_128 : R == 4 ? a.tw_n1 : a.tw_n2;
     constexpr int st = R == 4 ? 2 : 1;
-    const int s = a.tw_s, nlo = 1 << s, nhi = A::L >> s;
+    const int s = Q > 1 ? a.sub_s : a.tw_s, nlo = 1 << s, nhi = A::L >> s;
+    const float2 *src_lo = Q > 1 ? a.sub_lo : a.tw_lo, *src_hi = Q > 1 ? a.sub_hi : a.tw_hi;
     for (int i = threadIdx.x; i < 128; i += blockDim.x) t[i] = src128[i * st];
-    for (int i = threadIdx.x; i < nlo; i += blockDim.x) lo[i] = a.tw_lo[i];
-    for (int i = threadIdx.x; i < nhi; i += blockDim.x) hi[i] = a.tw_hi[i];
+    for (int i = threadIdx.x; i < nlo; i += blockDim.x) lo[i] = src_lo[i];
+    for (int i = threadIdx.x; i < nhi; i += blockDim.x) hi[i] = src_hi[i];
     __syncthreads();
     return AccTw{t, lo, hi, s};
 }
 
-// Ring offsets of block b's unit: tab[dd] = ((g0 + b - dd) mod FBS) L for delays dd < 256 (threads 0..255)
-template <int R>
+// Ring offsets of block b's unit: tab[dd] = ((g0 + b - dd) mod FBS) L for delays dd < 256 (threads 0..255);
+// L = Q R 2^14 is the ring's spectrum length
+template <int R, int Q = 1>
 __device__ __forceinline__ void acc_slot_table(const InvArgs &a, int b, uint32_t *tab)
 {
     if (threadIdx.x < 256) {
         int sl = (int)((a.g0 + b - (long long)threadIdx.x) % a.FBS);
         if (sl < 0) sl += a.FBS;
-        tab[threadIdx.x] = (uint32_t)sl * (uint32_t)(R * ACC_S);
+        tab[threadIdx.x] = (uint32_t)sl * (uint32_t)(Q * R * ACC_S);
     }
 }
 
@@ -97,14 +100,16 @@ __device__ __forceinline__ void acc_chirp_to_lds(const float2 *Hr, float2 *rows,
 }
 
 // The 16 byte delays of this thread for transform r of DM d: the bytes at r S + 16 p (p = k1 + 128 jtA),
-// the t-th one for bin position r S + p + 1024 t
-template <int R>
-__device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int r)
+// the t-th one for bin position r S + p + 1024 t; with Q > 1 sub-units, those of spectrum run s (the
+// bins s R S .. (s + 1) R S - 1) at s R S + r S + 16 p
+template <int R, int Q = 1>
+__device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int s, int r)
 {
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lt = tid & 255, p = (tid >> 8) * 32 + (lt & 31) + 128 * (lt >> 5);
-    return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (R * ACC_S) + (size_t)r * ACC_S + 16u * (uint32_t)p);
+    return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (Q * R * ACC_S) + (size_t)s * (R * ACC_S) +
+                                            (size_t)r * ACC_S + 16u * (uint32_t)p);
 }
 
 // Transform index of acc_unit's body: V >= 0 known at compile time, V = -1 a run-time r
@@ -118,8 +123,16 @@ struct AccR {
 // dl holds this unit's first delay words on entry and, on return, those of the next unit (DM dn, block
 // bn, when next): each transform's delays are loaded one transform ahead. slot: this unit's ring offsets
 // (acc_slot_table) on entry; the next unit's are written to slot_next before the unit's last barrier.
-template <int R, bool RING32, int VAR>
-__device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 *lds_all, const AccTw &tw,
+//
+// Q > 1 sub-units (lengths L = Q 2^15, R = 2): the unit (d, b) is split by decimation in frequency into Q
+// sub-units c, each the unit above of length L' = L / Q on the spectrum
+//   X_c[k'] = sum_s Y[k' + s L'] e^{+2 pi i c (k' + s L') / L},   k' in [0, L'),
+// whose kept outputs [L'/2, L') are y[M + Q u + c], u in [0, L'/2) (M = L / 2). The factor
+// e^{+2 pi i c k / L} rides in sub-unit c's chirp table (H_c = H e^{+2 pi i c k / L}), so the gather of
+// X_c reads the Q runs of L' bins with their own delays and chirps and sums them; a sub-unit keeps
+// L' / 2 outputs in registers. dl then runs through (transform r, run s) pairs, one ahead.
+template <int R, bool RING32, int VAR, int Q = 1>
+__device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, float2 *lds_all, const AccTw &tw,
                                          double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn,
                                          int bn, const uint32_t *slot, uint32_t *slot_next)
 {
@@ -127,8 +140,10 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
+    static_assert(Q == 1 || (R == 2 && !f16 && !tangent), "sub-units: R = 2, plain, uint16 or fold epilogues");
+    constexpr int MB = Q * M; // samples per block
     const bool fold = a.fold_sum != nullptr;
-    const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
+    const float2 *__restrict__ Hd = a.H_p + ((size_t)d * Q + c) * (size_t)(Q * L);
     const float2 *__restrict__ ring = a.ring_p;
     float power = 0.f;
     // R = 4: transform 0 has its own code (its combine factors are 1: the accumulators start as its
@@ -160,7 +175,7 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
         const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63; // wv: wave-uniform (SGPRs)
         float2 *rows = acc_wave_rows(lds_all, wv);
         float2 v[16];
-        {
+        if constexpr (Q == 1) {
             const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
             const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
 #pragma unroll
@@ -171,9 +186,9 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
             }
             if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Hd, rows, wv, lane);
             if (r + 1 < R)
-                dl = acc_delays<R>(a, d, r + 1);
+                dl = acc_delays<R>(a, d, 0, r + 1);
             else if (next)
-                dl = acc_delays<R>(a, dn, 0);
+                dl = acc_delays<R>(a, dn, 0, 0);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
 #pragma unroll
@@ -185,6 +200,58 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
                     float2 &y = v[2 * i + e];
                     y = cmul(y, hv[e]);
                     if constexpr (f16) power += y.x * y.x + y.y * y.y;
+                }
+            }
+        } else {
+            // run 0 into v (its chirp requested at the unit start for r = 0, else during the previous
+            // combine), then runs 1..Q-1 through w, each with its chirp in the wave's rows; the delays
+            // one (r, s) step ahead
+            const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
+            {
+                const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p0 + 1024u * t;
+                    v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
+                }
+                if (r == 0) acc_chirp_to_lds(Hd, rows, wv, lane);
+                dl = acc_delays<R, Q>(a, d, 1, r);
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float4 h = hrow[64 * i];
+                    v[2 * i] = cmul(v[2 * i], make_float2(h.x, h.y));
+                    v[2 * i + 1] = cmul(v[2 * i + 1], make_float2(h.z, h.w));
+                }
+            }
+#pragma unroll 1
+            for (int s = 1; s < Q; ++s) {
+                const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
+                const uint32_t ps = p0 + (uint32_t)s * (uint32_t)L;
+                float2 w[16];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + ps + 1024u * t;
+                    w[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
+                }
+                // the chirp after the ring-offset lookups (an LDS read issued behind a load into LDS waits
+                // for it), once this wave's reads of its rows are complete
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                acc_chirp_to_lds(Hd + (size_t)s * L + (size_t)r * S, rows, wv, lane);
+                if (s + 1 < Q)
+                    dl = acc_delays<R, Q>(a, d, s + 1, r);
+                else if (r + 1 < R)
+                    dl = acc_delays<R, Q>(a, d, 0, r + 1);
+                else if (next)
+                    dl = acc_delays<R, Q>(a, dn, 0, 0);
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float4 h = hrow[64 * i];
+                    v[2 * i] = cadd(v[2 * i], cmul(w[2 * i], make_float2(h.x, h.y)));
+                    v[2 * i + 1] = cadd(v[2 * i + 1], cmul(w[2 * i + 1], make_float2(h.z, h.w)));
                 }
             }
         }
@@ -317,29 +384,31 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
         inv_scale = 1.f / out_scale;
         if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
     }
-    float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * M : nullptr;
-    uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * M : nullptr;
-    uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * M : nullptr;
-    const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * M : nullptr;
+    float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * MB : nullptr;
+    uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * MB : nullptr;
+    uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * MB : nullptr;
+    const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * MB : nullptr;
     float *fb = reinterpret_cast<float *>(lds_all); // fold buffer: sample m at (m % MPT) FOLD_X + m / MPT
     // the fold bins of the samples this thread folds, [MPT tid, MPT tid + MPT), requested before the
     // outputs are written so that their latency overlaps that
     uint4 bw[MPT / 8];
     if (fold) {
-        const uint4 *bins4 = reinterpret_cast<const uint4 *>(a.bins + (size_t)b * M + (size_t)MPT * tid);
+        // sub-units: the bins of sub-unit c's samples are contiguous (launch_bins_sub)
+        const uint4 *bins4 = reinterpret_cast<const uint4 *>(a.bins + ((size_t)b * Q + c) * M + (size_t)MPT * tid);
 #pragma unroll
         for (int i = 0; i < MPT / 8; ++i) bw[i] = bins4[i];
     }
     auto out1 = [&](int m, float2 y) {
-        if (oc) st32(oc, (uint32_t)m, y);
-        if (ou) st32(ou, (uint32_t)m, (enc_u16(y.y) << 16) | enc_u16(y.x));
+        const uint32_t mo = Q == 1 ? (uint32_t)m : (uint32_t)(Q * m + c); // sample index in the block
+        if (oc) st32(oc, mo, y);
+        if (ou) st32(ou, mo, (enc_u16(y.y) << 16) | enc_u16(y.x));
         if constexpr (f16)
-            if (oh) st32(oh, (uint32_t)m, enc_f16_pair(y, inv_scale));
+            if (oh) st32(oh, mo, enc_f16_pair(y, inv_scale));
         if (fold) {
             // primal: |y|^2; tangent: 2 Re(conj(ref) y), NaN where the primal sample is skipped (|ref| = 0)
             float v = y.x * y.x + y.y * y.y;
             if constexpr (tangent) {
-                const float2 rf = ld32(tref, (uint32_t)m);
+                const float2 rf = ld32(tref, mo);
                 const float I = rf.x * rf.x + rf.y * rf.y;
                 v = I != 0.0f ? 2.f * (rf.x * y.x + rf.y * y.y) : __builtin_nanf("");
             }
@@ -400,7 +469,7 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, float2 
     }
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f;
-    if (next) acc_slot_table<R>(a, bn, slot_next);
+    if (next) acc_slot_table<R, Q>(a, bn, slot_next);
     if (fold) fold_flush(a, d, hsum, hcnt, tid, WG);
     __syncthreads();
 }

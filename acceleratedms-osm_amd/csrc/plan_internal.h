@@ -19,6 +19,8 @@ struct BinsArgs {
     long long n;
     unsigned long nbins;
     double t_start, dt, period;
+    int q = 1;       // > 1: in the order of a sub-unit accumulate launch of q sub-units (launch_bins_sub)
+    long long m = 0; //   samples per block
 };
 // every launch argument of one push
 struct PushArgs {
@@ -29,9 +31,11 @@ struct PushArgs {
     BinsArgs bins{};
     InvArgs inv{};
 };
-void plan_push_args(msosm_plan *p, const void *src_device, PushArgs *out);
+// allow_sub: the caller runs the plan's own dedispersion launch, which may be the sub-unit accumulate
+// form (the uneven grid's one-launch kernel and tangent pushes read the natural layouts)
+void plan_push_args(msosm_plan *p, const void *src_device, PushArgs *out, bool allow_sub);
 // forward transforms, overlap history and fold bins of one push; fills the dedispersion arguments
-int plan_push_prepare(msosm_plan *p, const void *src_device, InvArgs *out);
+int plan_push_prepare(msosm_plan *p, const void *src_device, InvArgs *out, bool allow_sub);
 // advance the ring position and the fold clock after the dedispersion launch
 void plan_push_commit(msosm_plan *p);
 // MSOSM_OK when every DM has a design and an output or fold is selected (sets the error otherwise)

@@ -1,0 +1,148 @@
+"""GPU parity of the sub-unit accumulate form (FFT lengths 2^17 and 2^18) against the FP64 oracle.
+
+At these lengths a unit's kept outputs do not fit a CU, so the accumulate launch splits each unit
+(DM, block) by decimation in frequency into Q = L / 2^15 sub-units: sub-unit c dedisperses the
+spectrum X_c[k'] = sum_s Y[k' + s 2^15] e^{+2 pi i c (k' + s 2^15) / L} with a 2^15-point accumulate
+transform and keeps the outputs y[M + Q u + c]. Another factorisation again, so its bar is the
+oracle's (rel-L2 1e-5, fold counts bit-exact), agreement with the two-pass form to FP32 rounding, and
+bit-identity with itself under every workgroup count and launch form.
+"""
+import numpy as np
+import pytest
+
+import msosm_amd as ms
+from oracle import oracle as orc
+from tests.util import check_u16_codes, make_input, oracle_design, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+# band 784-816 MHz: the reference's minimum order gives FFT length 2^17 for DM 127..505 and 2^18 for
+# DM 506..2020 (delaycount <= 128)
+BW, F0 = 32e6, 784e6
+DMS = {17: [150.0, 320.0, 500.0], 18: [520.0, 1000.0, 1500.0]}
+ACC = ms.ORDER_XCD_DM_MAJOR | ms.SCHEDULE_ACCUM
+
+
+def _design(log2L):
+    d = ms.Design(BW, F0, np.asarray(DMS[log2L], dtype=np.float32))
+    assert 2 * d.M == 1 << log2L and d.delaycount <= 256, (log2L, d.M, d.delaycount)
+    return d
+
+
+def _run(design, raw, fmt, batch, out, wg=0, order=ACC, mode=ms.LAUNCH_PERSISTENT, fold=None):
+    per = batch * design.M
+    plan = ms.Plan.from_design(design, batch, in_format=fmt, out_flags=out)
+    plan.set_schedule(wg, order)
+    plan.set_launch_mode(mode)
+    if fold is not None:
+        plan.attach_fold(fold)
+    outs = []
+    for c in range(raw.shape[0] // per):
+        plan.push(raw[c * per:(c + 1) * per])
+        if out & ms.OUT_C64:
+            outs.append(plan.output_c64())
+        elif out & ms.OUT_U16:
+            outs.append(plan.output_u16())
+    plan.synchronize()
+    plan.close()
+    return np.concatenate(outs, axis=1) if outs else None
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+def test_sub_accum_series_and_fold_match_oracle(log2L):
+    design = _design(log2L)
+    batch, calls, nbins, period = 2, 3, 512, 0.0011
+    per = batch * design.M
+    n = per * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, float(design.dms[1]), period, seed=80 + log2L)
+    raw, x = raw[:n], x[:n]
+    print(f"case sub-unit accumulate L=2^{log2L} dms={list(design.dms)} dc={design.delaycount}", flush=True)
+    fold = ms.Fold(period, BW, per, nbins, num_series=len(design.dms))
+    got = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64, fold=fold)
+    _, sums, cnts = fold.read()
+    fold.close()
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    for d in range(len(design.dms)):
+        err = rel_l2(got[d], ref[d])
+        rs, rc = np.zeros(nbins), np.zeros(nbins, dtype=np.uint64)
+        t0 = 0.0
+        for c in range(calls):
+            orc.fold(np.abs(ref[d, c * per:(c + 1) * per]) ** 2, nbins, t0, 1.0 / BW, period, rs, rc)
+            t0 += float(per) * (1.0 / BW)
+        srel = float(np.max(np.abs(sums[d] - rs) / rs))
+        print(f"  DM {design.dms[d]}: rel-L2 {err:.3e}, fold counts equal {np.array_equal(cnts[d], rc)}, "
+              f"sum rel err {srel:.2e}", flush=True)
+        assert err < TOL
+        assert np.array_equal(cnts[d], rc)
+        assert srel < TOL
+
+
+def test_sub_accum_uint16_output_codes():
+    design = _design(17)
+    batch, calls = 2, 2
+    per = batch * design.M
+    n = per * calls
+    raw, x = make_input(ms.IN_U16, 1 << (n - 1).bit_length(), BW, F0, 320.0, 0.0011, seed=91)
+    raw, x = raw[:n], x[:n]
+    print("case sub-unit accumulate uint16 L=2^17", flush=True)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    codes = _run(design, raw, ms.IN_U16, batch, ms.OUT_U16)
+    frac, worst = check_u16_codes(codes, ref)
+    print(f"  uint16: max code diff {worst}, frac differing (all at ties) {frac:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+def test_sub_accum_bit_identical_across_workgroups_and_launch_forms(log2L):
+    # 24 trials x 2 blocks x Q sub-units: items chained through few workgroups, on all CUs, on half, and
+    # in the one-launch-per-DM form; all give the same bits, and the two-pass form agrees to rounding
+    dms = np.linspace(DMS[log2L][0], DMS[log2L][-1], 24).astype(np.float32)
+    design = ms.Design(BW, F0, dms, M=1 << (log2L - 1))
+    batch, calls = 2, 2
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, float(dms[12]), 0.0011, seed=95 + log2L)
+    raw = np.ascontiguousarray(raw[:n])
+    ref = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64)
+    for wg, order, mode in [(8, ACC, ms.LAUNCH_PERSISTENT), (128, ACC, ms.LAUNCH_PERSISTENT),
+                            (0, ms.ORDER_LINEAR | ms.SCHEDULE_ACCUM, ms.LAUNCH_PERSISTENT),
+                            (0, ACC, ms.LAUNCH_PER_DM)]:
+        print(f"case sub-unit accumulate L=2^{log2L} workgroups={wg} order={order} mode={mode}", flush=True)
+        got = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64, wg=wg, order=order, mode=mode)
+        assert np.array_equal(got, ref)
+    seq = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64, order=ms.ORDER_XCD_DM_MAJOR)
+    err = rel_l2(seq, ref)
+    print(f"  two-pass vs sub-unit accumulate rel-L2 {err:.2e}", flush=True)
+    assert err < 2e-6
+
+
+def test_sub_accum_schedule_falls_back_for_tangent_pushes():
+    # the sub-unit form has no tangent epilogue: a tangent push under the accumulate schedule runs the
+    # two-pass form, bit for bit as a plan scheduled for it
+    design = _design(17)
+    batch, calls, nbins, period = 2, 2, 128, 0.0011
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, 320.0, period, seed=97)
+    raw = np.ascontiguousarray(raw[:n])
+    res = []
+    for order in (ACC, ms.ORDER_XCD_DM_MAJOR):
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_NONE)
+        plan.set_schedule(0, order)
+        for i, dm in enumerate(design.dms):
+            plan.set_dm_tangent(i, ms.design_transfer_function_ddm(BW, F0, float(dm), design.M))
+        fold = ms.Fold(period, BW, per, nbins, num_series=len(design.dms))
+        tfold = ms.Fold(period, BW, per, nbins, num_series=len(design.dms))
+        plan.attach_fold(fold)
+        for c in range(calls):
+            plan.push_tangent(raw[c * per:(c + 1) * per], tfold)
+        plan.synchronize()
+        res.append((fold.read(), tfold.read()))
+        plan.close()
+        fold.close()
+        tfold.close()
+    for (pa, ta), (pb, tb) in [(res[0], res[1])]:
+        for x, y in zip(pa + ta, pb + tb):
+            assert np.array_equal(x, y)
