@@ -94,8 +94,8 @@ def test_precision_stress_full_record_per_variant():
             # every trial: the FP16 pairs (times their block scale) against the complex64 series of the
             # same launch, on the device
             h = half_dev.float() * scale_dev
-            num = (h - c64_dev).norm(dim=(1, 2, 3))
-            den = c64_dev.norm(dim=(1, 2, 3))
+            num = (h - c64_dev).reshape(128, -1).norm(dim=1)
+            den = c64_dev.reshape(128, -1).norm(dim=1)
             worst = max(worst, float((num / den).max()))
             to_c = lambda t: t[pick].reshape(len(pick), per, 2).cpu().numpy().view(np.complex64)[..., 0]  # noqa: E731
             c64.append(to_c(c64_dev))
@@ -121,3 +121,48 @@ def test_precision_stress_full_record_per_variant():
     worst32 = max(rel_l2(res[ms.PHASE_FP32][0][j], ref[j]) for j in range(len(pick)))
     worst64 = max(rel_l2(res[ms.PHASE_FP64][0][j], ref[j]) for j in range(len(pick)))
     assert worst32 > 10 * worst64
+
+
+def test_dm_gradient_full_record():
+    # The refinement workload at its stated size: 2^24 complex64 samples (band 1392-1408 MHz, pulsar DM
+    # 123.456, P = 33 ms, 3 % FWHM, amplitude 0.5, AWGN sigma 1), 64 start DMs within +-0.5 of the truth,
+    # loss -sum((p - mean p)^2) / noise variance over 512-bin profiles. The derivative pass's gradient
+    # of every trial against central differences of the GPU loss (h = 2^-14: the loss varies on DM
+    # scales near 1e-3 here, see test_gpu_gradient.py), and of two trials against the FP64 oracle's
+    # gradient over the whole record.
+    from msosm_amd import autograd as ag
+
+    bw, f0, true, period, nbins, batch, n = 16e6, 1392e6, 123.456, 0.033, 512, 32, 1 << 24
+    x = orc.pulsar(n, bw, f0, true, period, duty=0.03, amp=0.5, sigma=1.0, seed=2024)
+    rec = torch.from_numpy(x.astype(np.complex64)).cuda()
+    rng = np.random.default_rng(20261203)
+    starts = (true + rng.uniform(-0.5, 0.5, 64)).astype(np.float32)
+    eng = ag.ProfileEngine(bw, f0, starts, batch, nbins, period, dm_max=float(np.float32(true + 0.5)))
+    M = eng.M
+    print(f"case DM gradient full record: n={n} M={M} dc={eng.delaycount} trials=64", flush=True)
+    nv = ag.profile_noise_var(rec, n, nbins)
+    dm = torch.tensor(starts.astype(np.float64), requires_grad=True)
+    ag.snr_loss(ag.folded_profile(dm, eng, rec), nv).sum().backward()
+    g = dm.grad.numpy()
+    h = 2.0 ** -14
+    with torch.no_grad():
+        lp = ag.snr_loss(ag.folded_profile(torch.tensor(starts + h, dtype=torch.float64), eng, rec), nv)
+        lm = ag.snr_loss(ag.folded_profile(torch.tensor(starts - h, dtype=torch.float64), eng, rec), nv)
+    fd = ((lp - lm) / (2 * h)).numpy()
+    rel = np.abs(g - fd) / np.maximum(np.abs(g), 1e-12)
+    print(f"  gradient vs central differences over 64 trials: median rel diff {np.median(rel):.2e}, "
+          f"max {rel.max():.2e} (|g| median {np.median(np.abs(g)):.3e})", flush=True)
+    assert np.median(rel) < 2e-2
+    pick = [0, 37]
+    xx = x.astype(np.complex64).astype(np.complex128)
+    delays = np.stack([orc.segmentation(bw, f0, float(starts[i]), M)[0] for i in pick])
+    H = np.stack([orc.transfer_function(bw, f0, float(starts[i]), M) for i in pick])
+    dH = np.stack([orc.transfer_function_ddm(bw, f0, float(starts[i]), M) for i in pick])
+    print("  FP64 oracle over the whole record (2 trials) ...", flush=True)
+    y, dy = orc.msosm(xx, M, delays, H), orc.msosm(xx, M, delays, dH)
+    for j, i in enumerate(pick):
+        p, dp, _ = orc.profile_tangent(y[j], dy[j], nbins, 0.0, 1.0 / bw, period, call_len=batch * M)
+        ref = -2.0 * np.sum((p - p.mean()) * dp) / nv
+        print(f"  DM {starts[i]:.6f}: gradient {g[i]:.6e} oracle {ref:.6e} central difference {fd[i]:.6e}",
+              flush=True)
+        assert abs(g[i] - ref) <= 1e-4 * abs(ref) + 1e-9
