@@ -354,10 +354,12 @@ struct msosm_plan {
     // same array when R = 1), H and dH in its chirp layout (acc_chirp_pos) and the delays as bytes in
     // thread order
     bool accum_ok = false;
-    int perm_rl = 0, perm_sl = 0, perm_ql = 0;
-    // lengths 2^17, 2^18: the accumulate launch runs acc_q sub-units of 2^15 points per unit, with the
-    // sub-unit length's twiddle tables at these offsets of tw (acc_q = 1: one unit)
+    int perm_rl = 0, perm_sl = 0;
+    // lengths 2^17, 2^18: the accumulate launch runs acc_q sub-units of 2^16 points per unit on the
+    // sub-unit spectra a gathering pass writes to xpre, with the sub-unit length's twiddle tables at
+    // these offsets of tw (acc_q = 1: one unit, no gathering pass)
     int acc_q = 1, tw_sub_lo = 0, tw_sub_hi = 0, tw_sub_128 = 0, tw_sub_s = 0;
+    float2 *xpre = nullptr;
     float2 *ring_p = nullptr, *H_p = nullptr, *Hdot_p = nullptr;
     uint8_t *delay_acc = nullptr;
     int sched_wg = 0, order = MSOSM_ORDER_XCD_DM_MAJOR; // tile schedule (0 workgroups = G)
@@ -383,7 +385,7 @@ static void plan_free(msosm_plan *p)
     if (p->out_stream) (void)hipStreamSynchronize(p->out_stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
                     p->bins, p->out_u16b[0], p->out_u16b[1], p->out_f16, p->out_scale, p->Hdot, p->tan_y, p->ring_p,
-                    p->H_p, p->Hdot_p, p->delay_acc};
+                    p->H_p, p->Hdot_p, p->delay_acc, p->xpre};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : {p->in_ev[0], p->in_ev[1], p->in_done[0], p->in_done[1], p->out_ready[0], p->out_ready[1],
@@ -462,9 +464,9 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     const int acc_lds = p->accum_ok ? accum_lds_bytes(log2L) : 0;
     if (p->accum_ok) {
         p->acc_q = acc_q;
-        p->perm_rl = acc_q > 1 ? 1 : log2L - 14;
+        // sub-unit plans read the natural ring, delays and chirps (in the gathering pass)
+        p->perm_rl = acc_q > 1 ? 0 : log2L - 14;
         p->perm_sl = 14;
-        p->perm_ql = acc_q > 1 ? 15 : log2L;
     }
     p->max_fold_bins = std::min(2048, (max_lds - std::max(tiles_lds, acc_lds) - INV_EXTRA_LDS) / 12);
     e = configure_lds(log2L, tiles_lds, tiles_lds + p->max_fold_bins * 12 + INV_EXTRA_LDS);
@@ -495,12 +497,12 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     p->tw_hi = n1 + n2 + nlo;
     int ntw = n1 + n2 + nlo + nhi;
     if (p->acc_q > 1) {
-        // the sub-unit length 2^15: lo (256), hi (128) split at 8 bits, and the 128-point roots
+        // the sub-unit length 2^16: lo and hi (256 each) split at 8 bits, and the 128-point roots
         p->tw_sub_s = 8;
         p->tw_sub_lo = ntw;
         p->tw_sub_hi = ntw + 256;
-        p->tw_sub_128 = ntw + 256 + 128;
-        ntw += 256 + 128 + 128;
+        p->tw_sub_128 = ntw + 512;
+        ntw += 256 + 256 + 128;
     }
     std::vector<float2> tw(ntw);
     auto put = [&](int off, long k, long n) {
@@ -517,10 +519,10 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         tw[p->tw_hi + k] = make_float2((float)re, (float)im);
     }
     if (p->acc_q > 1) {
-        for (int k = 0; k < 256; ++k) put(p->tw_sub_lo, k, 1l << 15);
-        for (int k = 0; k < 128; ++k) {
+        for (int k = 0; k < 256; ++k) put(p->tw_sub_lo, k, 1l << 16);
+        for (int k = 0; k < 256; ++k) {
             double re, im;
-            unit_root((long)k << 8, 1l << 15, &re, &im);
+            unit_root((long)k << 8, 1l << 16, &re, &im);
             tw[p->tw_sub_hi + k] = make_float2((float)re, (float)im);
         }
         for (int k = 0; k < 128; ++k) put(p->tw_sub_128, k, 128);
@@ -540,14 +542,16 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (e == hipSuccess) e = dalloc(p, &p->H, D * L * sizeof(float2));
     if (e == hipSuccess) e = dalloc(p, &p->delay, D * L * sizeof(uint16_t));
     if (e == hipSuccess) e = dalloc(p, &p->scratch, (size_t)p->G * L * sizeof(float2));
-    if (e == hipSuccess && p->accum_ok) {
+    if (e == hipSuccess && p->accum_ok && p->acc_q > 1) {
+        // the sub-unit spectra of one push: [D][B][acc_q][L / acc_q] complex64
+        e = dalloc(p, &p->xpre, D * B * L * sizeof(float2));
+    } else if (e == hipSuccess && p->accum_ok) {
         e = dalloc(p, &p->delay_acc, D * L);
         if (e == hipSuccess) e = hipMemset(p->delay_acc, 0, D * L);
         if (e == hipSuccess && p->perm_rl) e = dalloc(p, &p->ring_p, (size_t)p->FBS * L * sizeof(float2));
         if (e == hipSuccess && p->perm_rl) e = hipMemset(p->ring_p, 0, (size_t)p->FBS * L * sizeof(float2));
-        // per DM: one chirp table, or one per sub-unit
-        if (e == hipSuccess) e = dalloc(p, &p->H_p, D * p->acc_q * L * sizeof(float2));
-        if (e == hipSuccess) e = hipMemset(p->H_p, 0, D * p->acc_q * L * sizeof(float2));
+        if (e == hipSuccess) e = dalloc(p, &p->H_p, D * L * sizeof(float2));
+        if (e == hipSuccess) e = hipMemset(p->H_p, 0, D * L * sizeof(float2));
     }
     if (e == hipSuccess && (p->out_flags & MSOSM_OUT_C64)) e = dalloc(p, &p->out_c64, D * B * M * sizeof(float2));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->d2h_c64, hipEventDisableTiming);
@@ -578,10 +582,9 @@ extern "C" int msosm_plan_destroy(msosm_plan *p)
 }
 
 // copy L complex values to the device multiplied by 1/L (exact for normal values: a power of two), in
-// natural order and, for a plan with the accumulate launch, also in its chirp layout (bin k at
-// acc_chirp_pos(perm_pos(k))), or as the sub-unit chirp tables (launch_sub_chirp, on the plan's stream)
-// (dst_p holds q tables of L values: q = p->acc_q for H, 1 for dH/dDM, which has no sub-unit tables)
-static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src, int q)
+// natural order and, for a plan with the accumulate launch of one unit, also in its chirp layout (bin k
+// at acc_chirp_pos(perm_pos(k)))
+static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p, const float *src)
 {
     const int L = p->L;
     std::vector<float2> tmp((size_t)L);
@@ -589,10 +592,6 @@ static hipError_t upload_scaled(const msosm_plan *p, float2 *dst, float2 *dst_p,
     for (int k = 0; k < L; ++k) tmp[k] = make_float2(src[2 * k] * s, src[2 * k + 1] * s);
     hipError_t e = hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess || !dst_p) return e;
-    if (q > 1) {
-        e = launch_sub_chirp(dst, dst_p, p->log2L, q, p->stream);
-        return e == hipSuccess ? hipStreamSynchronize(p->stream) : e;
-    }
     std::vector<float2> perm((size_t)L);
     for (int k = 0; k < L; ++k) perm[acc_chirp_pos(perm_pos((uint32_t)k, p->perm_rl, p->perm_sl))] = tmp[k];
     return hipMemcpy(dst_p, perm.data(), perm.size() * sizeof(float2), hipMemcpyHostToDevice);
@@ -614,18 +613,15 @@ extern "C" int msosm_plan_set_dm(msosm_plan *p, int i, const int *delay_points, 
     HIPCHK(hipStreamSynchronize(p->stream));
     HIPCHK(hipMemcpy(p->delay + (size_t)i * p->L, dl.data(), p->L * sizeof(uint16_t), hipMemcpyHostToDevice));
     // the device copy carries the inverse transform's 1/L (exact: a power of two)
-    HIPCHK(upload_scaled(p, p->H + (size_t)i * p->L, p->H_p ? p->H_p + (size_t)i * p->acc_q * p->L : nullptr, H,
-                         p->acc_q));
-    if (p->accum_ok) {
-        // byte delays in thread order: position r S + p + 1024 t of a run (perm_pos_sub; one run of L bins,
-        // or one per sub-unit spectrum) goes to byte r S + 16 p + t of that run
+    HIPCHK(upload_scaled(p, p->H + (size_t)i * p->L, p->H_p ? p->H_p + (size_t)i * p->L : nullptr, H));
+    if (p->delay_acc) {
+        // byte delays in thread order: position r S + p + 1024 t (perm_pos) goes to byte r S + 16 p + t
         std::vector<uint8_t> db((size_t)p->L);
         const uint32_t S = 1u << 14;
         for (int k = 0; k < p->L; ++k) {
-            const uint32_t pos = perm_pos_sub((uint32_t)k, p->perm_rl, p->perm_sl, p->perm_ql);
-            const uint32_t run = pos >> p->perm_ql << p->perm_ql, w = pos - run;
-            const uint32_t r = w / S, q = w % S;
-            db[(size_t)run + (size_t)r * S + 16 * (q & 1023u) + (q >> 10)] = (uint8_t)delay_points[k];
+            const uint32_t pos = perm_pos((uint32_t)k, p->perm_rl, p->perm_sl);
+            const uint32_t r = pos / S, q = pos % S;
+            db[(size_t)r * S + 16 * (q & 1023u) + (q >> 10)] = (uint8_t)delay_points[k];
         }
         HIPCHK(hipMemcpy(p->delay_acc + (size_t)i * p->L, db.data(), db.size(), hipMemcpyHostToDevice));
     }
@@ -653,7 +649,7 @@ extern "C" int msosm_plan_set_dm_tangent(msosm_plan *p, int i, const float *dH)
             return set_error(MSOSM_ERR_ALLOC, std::string("plan_set_dm_tangent: device allocation: ") + hipGetErrorString(e));
         p->tan_set.assign(p->D, 0);
     }
-    HIPCHK(upload_scaled(p, p->Hdot + (size_t)i * p->L, p->Hdot_p ? p->Hdot_p + (size_t)i * p->L : nullptr, dH, 1));
+    HIPCHK(upload_scaled(p, p->Hdot + (size_t)i * p->L, p->Hdot_p ? p->Hdot_p + (size_t)i * p->L : nullptr, dH));
     p->tan_set[i] = 1;
     return MSOSM_OK;
 }
@@ -713,7 +709,6 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out, bool a
     fa.ring_p = p->ring_p; // a second copy only where the accumulate layout differs
     fa.perm_rl = p->perm_rl;
     fa.perm_sl = p->perm_sl;
-    fa.perm_ql = p->perm_ql;
     a.log2L = p->log2L;
     a.fwd_grid = std::min(p->B, p->G);
     a.hist = HistArgs{p->hist, src, p->in_fmt, (long long)(p->B - 1) * p->M, p->M};
@@ -748,6 +743,7 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out, bool a
         ia.sub_hi = tw + p->tw_sub_hi;
         ia.sub_128 = tw + p->tw_sub_128;
         ia.sub_s = p->tw_sub_s;
+        ia.xpre = p->xpre;
     }
     if (p->fold) {
         msosm_fold *f = p->fold;
@@ -879,7 +875,8 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
                 one.D = 1;
                 one.H = x.H + (size_t)d * p->L;
                 one.delay = x.delay + (size_t)d * p->L;
-                one.H_p = x.H_p + (size_t)d * p->acc_q * p->L;
+                one.H_p = x.H_p + (size_t)d * p->L;
+                if (x.xpre) one.xpre = x.xpre + (size_t)d * p->B * p->L;
                 if (x.delay_acc) one.delay_acc = x.delay_acc + (size_t)d * p->L;
                 if (x.out_c64) one.out_c64 = x.out_c64 + d * ser;
                 if (x.out_u16) one.out_u16 = x.out_u16 + d * ser;

@@ -19,9 +19,8 @@ struct FwdArgs {
     float2 *scratch;       // [grid][L]
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
-    float2 *ring_p;        // [FBS][L] the same spectra in the accumulate layout (perm_pos_sub), or null
-    int perm_rl, perm_sl;  //   its bin k at ((k mod 2^rl) << sl) | (k >> rl) within each run of 2^ql bins
-    int perm_ql;
+    float2 *ring_p;        // [FBS][L] the same spectra in the accumulate layout (perm_pos), or null
+    int perm_rl, perm_sl;  //   its bin k at ((k mod 2^rl) << sl) | (k >> rl)
 };
 
 // Position of spectrum bin k in the accumulate layout of a plan's ring and chirp tables:
@@ -32,13 +31,6 @@ __host__ __device__ __forceinline__ uint32_t perm_pos(uint32_t k, int rl, int sl
     return ((k & ((1u << rl) - 1u)) << sl) | (k >> rl);
 }
 
-// The accumulate layout of lengths L = Q 2^ql made of Q sub-unit spectra (Q = 1: perm_pos): bin k keeps
-// its run of 2^ql bins and takes position perm_pos of its index within the run.
-__host__ __device__ __forceinline__ uint32_t perm_pos_sub(uint32_t k, int rl, int sl, int ql)
-{
-    const uint32_t lo = k & ((1u << ql) - 1u);
-    return (k - lo) | perm_pos(lo, rl, sl);
-}
 
 // Index of accumulate-layout position P = r S + p + 1024 t (S = 2^14; p in [0, 1024) the column index
 // of the transform's thread tid = 256 (p mod 128 div 32) + p mod 32 + 32 (p div 128)) in the accumulate
@@ -78,12 +70,13 @@ struct InvArgs {
     // [0, 1024)), the t-th one being bin position r 2^14 + p + 1024 t
     const float2 *ring_p, *H_p;
     const uint8_t *delay_acc;
-    // sub-unit accumulate launch (lengths 2^17, 2^18: Q = L / 2^15 sub-units of 2^15 points per unit):
-    // the twiddle tables of the sub-unit length (lo, hi split at sub_s; the 128-point roots); H_p then
-    // holds, per DM, Q tables H_c[k] = H[k] e^{+2 pi i c k / L} (c = 0..Q-1), each in the layout
-    // perm_pos_sub + acc_chirp_pos per sub-unit spectrum
+    // sub-unit accumulate launch (lengths 2^17, 2^18: Q = L / 2^16 sub-units of 2^16 points per unit):
+    // the twiddle tables of the sub-unit length (lo, hi split at sub_s; the 128-point roots), and the
+    // sub-unit spectra [D][B][Q][2^16] that the gathering pass writes (sub-unit c of unit (d, b) at
+    // ((d B + b) Q + c) 2^16, in the accumulate layout perm_pos with R = 4)
     const float2 *sub_lo, *sub_hi, *sub_128;
     int sub_s;
+    float2 *xpre;
 };
 
 // One persistent launch over several FFT lengths ("length classes"): class c owns the units
@@ -146,7 +139,8 @@ void split_length(int log2L, int *n1, int *n2);
 bool length_supported(int log2L);
 bool split_supported(int log2L); // a role-split dedispersion kernel exists for this length
 // accumulate form (kernels_accum.hip): lengths 2^14..2^16 (spectrum layout R = L / 2^14), and 2^17, 2^18
-// as Q = accum_subunits sub-units of 2^15 points each (the plain and uint16/complex64/fold epilogues)
+// as Q = accum_subunits sub-units of 2^16 points each (a gathering pass, then the sub-unit transforms;
+// the plain and uint16/complex64/fold epilogues)
 bool accum_supported(int log2L);
 int accum_subunits(int log2L);
 int accum_lds_bytes(int log2L);
@@ -165,9 +159,7 @@ hipError_t launch_bins(uint16_t *bins, long long n, unsigned long nbins, double 
 // the bins of a sub-unit accumulate launch: sample m = Q j + c of block b at (b Q + c) (M / Q) + j
 hipError_t launch_bins_sub(uint16_t *bins, long long n, unsigned long nbins, double t_start, double dt, double period,
                            int q, long long m, hipStream_t st);
-// chirp tables of a sub-unit accumulate plan for one DM: from H (natural order, L values) the Q tables
-// H_c[k] = H[k] e^{+2 pi i c k / L}, the twiddle evaluated in FP64, each in the sub-unit chirp layout
-hipError_t launch_sub_chirp(const float2 *H, float2 *Hs, int log2L, int q, hipStream_t st);
+
 hipError_t launch_fold_series(const float2 *a, const float2 *b, long long size, long long stride, int nseries,
                               unsigned long nbins, double t_start, double dt, double period, double *sums,
                               unsigned long long *cnts, hipStream_t st);

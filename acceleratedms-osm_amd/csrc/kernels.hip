@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
                 const uint32_t k = (uint32_t)(tile * NC + row + N2 * n1);
                 const float2 y = lds[lds_pad(lt) + 272 * i]; // pad(lt + 256 i)
                 st32(dst, k, y);
-                if (dstp) st32(dstp, perm_pos_sub(k, a.perm_rl, a.perm_sl, a.perm_ql), y);
+                if (dstp) st32(dstp, perm_pos(k, a.perm_rl, a.perm_sl), y);
             }
         };
         pass_b<N1, N2, -1>(scr, prep, emit, lds_all, tw);
@@ -146,26 +146,6 @@ __global__ void bins_sub_kernel(uint16_t *bins, long long n, unsigned long nbins
     if (i >= n) return;
     const long long b = i / m, mm = i - b * m, j = mm / q, c = mm - j * q;
     bins[b * m + c * (m / q) + j] = (uint16_t)fold_bin(i, nbins, t_start, dt, period);
-}
-
-// Chirp tables of a sub-unit accumulate plan for one DM (Q sub-units of 2^15 points, L = 2^15 Q):
-// Hs[c L + run(k) + acc_chirp_pos(perm_pos(k mod 2^15, 1, 14))] = H[k] e^{+2 pi i (c k mod L) / L}, with
-// the twiddle and the product in FP64 and one rounding to complex64
-__global__ void sub_chirp_kernel(const float2 *H, float2 *Hs, int log2L, int q)
-{
-    const long long L = 1ll << log2L;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L * q) return;
-    const long long c = i >> log2L, k = i & (L - 1);
-    const long long mphase = (c * k) & (L - 1);
-    double sn, cs;
-    sincospi(2.0 * (double)mphase / (double)L, &sn, &cs);
-    const float2 h = H[k];
-    const double re = (double)h.x * cs - (double)h.y * sn, im = (double)h.x * sn + (double)h.y * cs;
-    constexpr uint32_t LS = 1u << 15;
-    const uint32_t kk = (uint32_t)k;
-    const uint32_t pos = (kk & ~(LS - 1u)) + acc_chirp_pos(perm_pos(kk & (LS - 1u), 1, 14));
-    Hs[c * L + pos] = make_float2((float)re, (float)im);
 }
 
 // Unfused fold of series (Fold_GPU::calculate_intensity + fold_data_phase, fold_gpu.cpp:27-54):
@@ -330,7 +310,7 @@ hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st)
 
 hipError_t launch_inv(int log2L, const InvArgs &a, int grid, hipStream_t st)
 {
-    if ((a.order & MSOSM_SCHEDULE_ACCUM) && accum_supported(log2L) && a.delay_acc)
+    if ((a.order & MSOSM_SCHEDULE_ACCUM) && accum_supported(log2L) && (a.delay_acc || a.xpre))
         return launch_accum(log2L, a, grid, st);
     switch (log2L) {
 #define X(lg, a_, b_)                                                                                                  \
@@ -360,13 +340,6 @@ hipError_t launch_bins_sub(uint16_t *bins, long long n, unsigned long nbins, dou
 {
     hipLaunchKernelGGL(bins_sub_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bins, n, nbins, t_start,
                        dt, period, q, m);
-    return hipGetLastError();
-}
-
-hipError_t launch_sub_chirp(const float2 *H, float2 *Hs, int log2L, int q, hipStream_t st)
-{
-    const long long n = (long long)q << log2L;
-    hipLaunchKernelGGL(sub_chirp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, H, Hs, log2L, q);
     return hipGetLastError();
 }
 

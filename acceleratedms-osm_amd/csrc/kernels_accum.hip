@@ -1,7 +1,8 @@
 // Persistent dedispersion kernels of the accumulate form: one workgroup per CU walks (DM, block) units
-// in the plan's unit order (FFT lengths 2^14, 2^15, 2^16), or (DM, block, sub-unit) items (lengths 2^17,
-// 2^18: Q = L / 2^15 sub-units of 2^15 points); each unit's inverse transform stays in the CU (LDS and
-// registers), so the launch reads the gathered spectra and writes only its outputs.
+// in the plan's unit order (FFT lengths 2^14, 2^15, 2^16), or (DM, block, sub-unit) items after a
+// gathering pass (lengths 2^17, 2^18: Q = L / 2^16 sub-units of 2^16 points); each unit's inverse
+// transform stays in the CU (LDS and registers), so the launch reads the gathered spectra and writes
+// only its outputs.
 #include "msosm_accum.h"
 
 namespace msosm {
@@ -24,7 +25,7 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
     // the workgroup's units in the plan's order, each unit's first delays and ring offsets prepared
     // during the previous one
-    uint4 dl = have ? acc_delays<R>(a, d, 0, 0) : make_uint4(0u, 0u, 0u, 0u);
+    uint4 dl = have ? acc_delays<R>(a, d, 0) : make_uint4(0u, 0u, 0u, 0u);
     int cur = 0;
 #pragma unroll 1
     while (have) {
@@ -39,17 +40,67 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     }
 }
 
-// Sub-unit accumulate kernel (L = Q 2^15, plain epilogue: complex64 / uint16 outputs, fused fold). Items
-// x = (d B + b) Q + c in the XCD-grouped order: the G/8 workgroups of an XCD take consecutive items, so
-// they run the Q sub-units of a block and the blocks of one DM together, sharing the blocks' ring values
-// and the DM's chirp tables in their L2.
+// Gathering pass of the sub-unit accumulate launch (L = Q 2^16): one 256-thread workgroup per
+// (DM d, chunk of 4096 sub-spectrum bins), over every block b of the push. For each k' it reads the Q
+// bins k = k' + s 2^16 (their delays and chirps once, the ring slot of each block's delay: Y_b[k]), and
+// writes the Q sub-unit spectra of each block
+//   X_c[k'] = e^{+2 pi i c k' / L} sum_s e^{+2 pi i c s / Q} Y_b[k' + s 2^16]   (a Q-point DFT over s)
+// at position perm_pos(k', 2, 14) of sub-unit c (the order in which acc_unit's transforms read them).
+// The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
+// factors) and its powers c = 2, 3 from products of it.
+template <int Q>
+__global__ void __launch_bounds__(256) acc_pre_kernel(InvArgs a)
+{
+    constexpr int LS = 1 << 16, CH = 4096;
+    constexpr long long L = (long long)Q * LS;
+    const int chunk = (int)(blockIdx.x % (LS / CH)), d = (int)(blockIdx.x / (LS / CH));
+    const uint16_t *__restrict__ dly = a.delay + (size_t)d * L;
+    const float2 *__restrict__ H = a.H + (size_t)d * L;
+    float2 *__restrict__ X = a.xpre + (size_t)d * a.B * (size_t)L;
+    const int mask = (1 << a.tw_s) - 1;
+#pragma unroll 1
+    for (int j = 0; j < CH / 256; ++j) {
+        const int kp = chunk * CH + j * 256 + (int)threadIdx.x;
+        int dd[Q];
+        float2 h[Q];
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            dd[s] = dly[kp + (long long)s * LS];
+            h[s] = H[kp + (long long)s * LS];
+        }
+        const float2 w1 = cmul(a.tw_hi[kp >> a.tw_s], a.tw_lo[kp & mask]);
+        float2 wc[Q];
+        wc[0] = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int c = 1; c < Q; ++c) wc[c] = c == 1 ? w1 : cmul(wc[c - 1], w1);
+        const uint32_t pos = perm_pos((uint32_t)kp, 2, 14);
+#pragma unroll 1
+        for (int b = 0; b < a.B; ++b) {
+            float2 y[Q];
+#pragma unroll
+            for (int s = 0; s < Q; ++s) {
+                long long sl = (a.g0 + b - (long long)dd[s]) % a.FBS;
+                if (sl < 0) sl += a.FBS;
+                y[s] = cmul(a.ring[sl * L + kp + (long long)s * LS], h[s]);
+            }
+            dft<+1, Q>(y);
+            float2 *Xb = X + (size_t)b * L;
+            Xb[pos] = y[0];
+#pragma unroll
+            for (int c = 1; c < Q; ++c) Xb[(size_t)c * LS + pos] = cmul(y[c], wc[c]);
+        }
+    }
+}
+
+// Sub-unit accumulate kernel (L = Q 2^16, plain epilogue: complex64 / uint16 outputs, fused fold) on the
+// gathering pass's spectra. Items x = (d B + b) Q + c in the XCD-grouped order: the G/8 workgroups of an
+// XCD take consecutive items.
 template <int Q, bool RING32>
 __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
 {
-    constexpr int R = 2;
+    constexpr int R = 4;
     using A = AccShape<R>;
     extern __shared__ float2 lds_all[];
-    uint32_t *slots = reinterpret_cast<uint32_t *>(lds_all + ACC_AREA + A::TW_WORDS); // two tables of 256
     const long long total = (long long)a.D * a.B * Q;
     const int G = gridDim.x, w = blockIdx.x;
     long long x = G % 8 == 0 ? (long long)(w % 8) * (G / 8) + w / 8 : w;
@@ -62,23 +113,20 @@ __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
     int d = 0, b = 0, c = 0;
     bool have = x < total; // uniform over the workgroup
     if (have) item(x, d, b, c);
-    if (have) acc_slot_table<R, Q>(a, b, slots); // made visible by the twiddle load's barrier
     const AccTw tw = acc_load_twiddles<R, Q>(lds_all, a);
     double *hsum = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_all) + A::LDS_BYTES);
     unsigned int *hcnt = reinterpret_cast<unsigned int *>(hsum + a.nbins);
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
-    uint4 dl = have ? acc_delays<R, Q>(a, d, 0, 0) : make_uint4(0u, 0u, 0u, 0u);
-    int cur = 0;
+    uint4 dl = make_uint4(0u, 0u, 0u, 0u); // unused by sub-units
 #pragma unroll 1
     while (have) {
         const long long xn = x + G;
         const bool next = xn < total;
         int dn = 0, bn = 0, cn = 0;
         if (next) item(xn, dn, bn, cn);
-        acc_unit<R, RING32, VAR_PLAIN, Q>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn,
-                                          slots + 256 * cur, slots + 256 * (cur ^ 1));
-        cur ^= 1;
+        acc_unit<R, RING32, VAR_PLAIN, Q>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, nullptr,
+                                          nullptr);
         x = xn;
         d = dn;
         b = bn;
@@ -87,9 +135,9 @@ __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
     }
 }
 
-static int acc_r(int log2L) { return log2L == 14 ? 1 : log2L == 15 ? 2 : log2L == 16 ? 4 : log2L <= 18 ? 2 : 0; }
+static int acc_r(int log2L) { return log2L == 14 ? 1 : log2L == 15 ? 2 : log2L == 16 || log2L == 17 || log2L == 18 ? 4 : 0; }
 
-int accum_subunits(int log2L) { return log2L == 17 ? 4 : log2L == 18 ? 8 : 1; }
+int accum_subunits(int log2L) { return log2L == 17 ? 2 : log2L == 18 ? 4 : 1; }
 
 bool accum_supported(int log2L) { return log2L >= 14 && log2L <= 18; }
 
@@ -123,8 +171,7 @@ static hipError_t configure_acc_t(int bytes)
 template <int Q>
 static hipError_t configure_acc_sub_t(int bytes)
 {
-    const void *fs[] = {reinterpret_cast<const void *>(&acc_sub_kernel<Q, true>),
-                        reinterpret_cast<const void *>(&acc_sub_kernel<Q, false>)};
+    const void *fs[] = {reinterpret_cast<const void *>(&acc_sub_kernel<Q, true>)};
     for (const void *f : fs) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -134,8 +181,8 @@ static hipError_t configure_acc_sub_t(int bytes)
 
 hipError_t configure_accum_lds(int log2L, int bytes)
 {
-    if (log2L == 17) return configure_acc_sub_t<4>(bytes);
-    if (log2L == 18) return configure_acc_sub_t<8>(bytes);
+    if (log2L == 17) return configure_acc_sub_t<2>(bytes);
+    if (log2L == 18) return configure_acc_sub_t<4>(bytes);
     switch (log2L) {
     case 14: return configure_acc_t<1>(bytes);
     case 15: return configure_acc_t<2>(bytes);
@@ -168,15 +215,15 @@ static hipError_t launch_acc_t(const InvArgs &a, int grid, hipStream_t st)
 template <int Q>
 static hipError_t launch_acc_sub_t(const InvArgs &a, int grid, hipStream_t st)
 {
-    using A = AccShape<2>;
+    using A = AccShape<4>;
     // the sub-unit form has the plain epilogue only; the caller chooses another form otherwise
-    if (a.tangent_ref || a.out_f16 || !a.sub_lo) return hipErrorInvalidValue;
+    if (a.tangent_ref || a.out_f16 || !a.sub_lo || !a.xpre) return hipErrorInvalidValue;
+    const unsigned pre_grid = (unsigned)((long long)a.D * ((1 << 16) / 4096));
+    hipLaunchKernelGGL((acc_pre_kernel<Q>), dim3(pre_grid), dim3(256), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     const size_t lds = A::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0) + INV_EXTRA_LDS;
-    const bool ring32 = (unsigned long long)a.FBS * (A::L * Q) * sizeof(float2) < (1ull << 32);
-    if (ring32)
-        hipLaunchKernelGGL((acc_sub_kernel<Q, true>), dim3(grid), dim3(WG), lds, st, a);
-    else
-        hipLaunchKernelGGL((acc_sub_kernel<Q, false>), dim3(grid), dim3(WG), lds, st, a);
+    hipLaunchKernelGGL((acc_sub_kernel<Q, true>), dim3(grid), dim3(WG), lds, st, a);
     return hipGetLastError();
 }
 
@@ -186,8 +233,8 @@ hipError_t launch_accum(int log2L, const InvArgs &a, int grid, hipStream_t st)
     case 14: return launch_acc_t<1>(a, grid, st);
     case 15: return launch_acc_t<2>(a, grid, st);
     case 16: return launch_acc_t<4>(a, grid, st);
-    case 17: return launch_acc_sub_t<4>(a, grid, st);
-    case 18: return launch_acc_sub_t<8>(a, grid, st);
+    case 17: return launch_acc_sub_t<2>(a, grid, st);
+    case 18: return launch_acc_sub_t<4>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

@@ -53,7 +53,7 @@ __device__ __forceinline__ AccTw acc_load_twiddles(float2 *lds_all, const InvArg
     float2 *lo = t + 128, *hi = lo + 256;
     // a 128-point table: the plan's N2 table has 128 entries for L <= 2^15, its N1 table 256 at 2^16
     const float2 *src128 = Q > 1 ? a.sub_128 : R == 4 ? a.tw_n1 : a.tw_n2;
-    constexpr int st = R == 4 ? 2 : 1;
+    constexpr int st = Q == 1 && R == 4 ? 2 : 1;
     const int s = Q > 1 ? a.sub_s : a.tw_s, nlo = 1 << s, nhi = A::L >> s;
     const float2 *src_lo = Q > 1 ? a.sub_lo : a.tw_lo, *src_hi = Q > 1 ? a.sub_hi : a.tw_hi;
     for (int i = threadIdx.x; i < 128; i += blockDim.x) t[i] = src128[i * st];
@@ -63,15 +63,14 @@ __device__ __forceinline__ AccTw acc_load_twiddles(float2 *lds_all, const InvArg
     return AccTw{t, lo, hi, s};
 }
 
-// Ring offsets of block b's unit: tab[dd] = ((g0 + b - dd) mod FBS) L for delays dd < 256 (threads 0..255);
-// L = Q R 2^14 is the ring's spectrum length
-template <int R, int Q = 1>
+// Ring offsets of block b's unit: tab[dd] = ((g0 + b - dd) mod FBS) L for delays dd < 256 (threads 0..255)
+template <int R>
 __device__ __forceinline__ void acc_slot_table(const InvArgs &a, int b, uint32_t *tab)
 {
     if (threadIdx.x < 256) {
         int sl = (int)((a.g0 + b - (long long)threadIdx.x) % a.FBS);
         if (sl < 0) sl += a.FBS;
-        tab[threadIdx.x] = (uint32_t)sl * (uint32_t)(Q * R * ACC_S);
+        tab[threadIdx.x] = (uint32_t)sl * (uint32_t)(R * ACC_S);
     }
 }
 
@@ -100,16 +99,14 @@ __device__ __forceinline__ void acc_chirp_to_lds(const float2 *Hr, float2 *rows,
 }
 
 // The 16 byte delays of this thread for transform r of DM d: the bytes at r S + 16 p (p = k1 + 128 jtA),
-// the t-th one for bin position r S + p + 1024 t; with Q > 1 sub-units, those of spectrum run s (the
-// bins s R S .. (s + 1) R S - 1) at s R S + r S + 16 p
-template <int R, int Q = 1>
-__device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int s, int r)
+// the t-th one for bin position r S + p + 1024 t
+template <int R>
+__device__ __forceinline__ uint4 acc_delays(const InvArgs &a, int d, int r)
 {
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lt = tid & 255, p = (tid >> 8) * 32 + (lt & 31) + 128 * (lt >> 5);
-    return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (Q * R * ACC_S) + (size_t)s * (R * ACC_S) +
-                                            (size_t)r * ACC_S + 16u * (uint32_t)p);
+    return *reinterpret_cast<const uint4 *>(a.delay_acc + (size_t)d * (R * ACC_S) + (size_t)r * ACC_S + 16u * (uint32_t)p);
 }
 
 // Transform index of acc_unit's body: V >= 0 known at compile time, V = -1 a run-time r
@@ -124,13 +121,12 @@ struct AccR {
 // bn, when next): each transform's delays are loaded one transform ahead. slot: this unit's ring offsets
 // (acc_slot_table) on entry; the next unit's are written to slot_next before the unit's last barrier.
 //
-// Q > 1 sub-units (lengths L = Q 2^15, R = 2): the unit (d, b) is split by decimation in frequency into Q
+// Q > 1 sub-units (lengths L = Q 2^16, R = 4): the unit (d, b) is split by decimation in frequency into Q
 // sub-units c, each the unit above of length L' = L / Q on the spectrum
 //   X_c[k'] = sum_s Y[k' + s L'] e^{+2 pi i c (k' + s L') / L},   k' in [0, L'),
-// whose kept outputs [L'/2, L') are y[M + Q u + c], u in [0, L'/2) (M = L / 2). The factor
-// e^{+2 pi i c k / L} rides in sub-unit c's chirp table (H_c = H e^{+2 pi i c k / L}), so the gather of
-// X_c reads the Q runs of L' bins with their own delays and chirps and sums them; a sub-unit keeps
-// L' / 2 outputs in registers. dl then runs through (transform r, run s) pairs, one ahead.
+// whose kept outputs [L'/2, L') are y[M + Q u + c], u in [0, L'/2) (M = L / 2). The gathering pass
+// (acc_pre_kernel) has written X_c in this unit's transform order (a.xpre), so each transform reads
+// its inputs contiguously: no delays, ring offsets or chirps here, and dl and slot are unused.
 template <int R, bool RING32, int VAR, int Q = 1>
 __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, float2 *lds_all, const AccTw &tw,
                                          double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn,
@@ -140,10 +136,10 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
     constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
-    static_assert(Q == 1 || (R == 2 && !f16 && !tangent), "sub-units: R = 2, plain, uint16 or fold epilogues");
+    static_assert(Q == 1 || (R == 4 && !f16 && !tangent), "sub-units: R = 4, plain, uint16 or fold epilogues");
     constexpr int MB = Q * M; // samples per block
     const bool fold = a.fold_sum != nullptr;
-    const float2 *__restrict__ Hd = a.H_p + ((size_t)d * Q + c) * (size_t)(Q * L);
+    const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring_p;
     float power = 0.f;
     // R = 4: transform 0 has its own code (its combine factors are 1: the accumulators start as its
@@ -186,9 +182,9 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
             }
             if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Hd, rows, wv, lane);
             if (r + 1 < R)
-                dl = acc_delays<R>(a, d, 0, r + 1);
+                dl = acc_delays<R>(a, d, r + 1);
             else if (next)
-                dl = acc_delays<R>(a, dn, 0, 0);
+                dl = acc_delays<R>(a, dn, 0);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
 #pragma unroll
@@ -203,57 +199,12 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
                 }
             }
         } else {
-            // run 0 into v (its chirp requested at the unit start for r = 0, else during the previous
-            // combine), then runs 1..Q-1 through w, each with its chirp in the wave's rows; the delays
-            // one (r, s) step ahead
+            // the sub-unit spectrum from the gathering pass, already in this transform's order
+            const float2 *__restrict__ X = a.xpre + (((size_t)d * a.B + b) * Q + c) * (size_t)L;
             const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
-            {
-                const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
 #pragma unroll
-                for (int t = 0; t < 16; ++t) {
-                    const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + p0 + 1024u * t;
-                    v[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
-                }
-                if (r == 0) acc_chirp_to_lds(Hd, rows, wv, lane);
-                dl = acc_delays<R, Q>(a, d, 1, r);
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const float4 h = hrow[64 * i];
-                    v[2 * i] = cmul(v[2 * i], make_float2(h.x, h.y));
-                    v[2 * i + 1] = cmul(v[2 * i + 1], make_float2(h.z, h.w));
-                }
-            }
-#pragma unroll 1
-            for (int s = 1; s < Q; ++s) {
-                const uint32_t dw[4] = {dl.x, dl.y, dl.z, dl.w};
-                const uint32_t ps = p0 + (uint32_t)s * (uint32_t)L;
-                float2 w[16];
-#pragma unroll
-                for (int t = 0; t < 16; ++t) {
-                    const uint32_t off = slot[(dw[t >> 2] >> (8 * (t & 3))) & 255u] + ps + 1024u * t;
-                    w[t] = RING32 ? ld32(ring, off) : ring[(size_t)off];
-                }
-                // the chirp after the ring-offset lookups (an LDS read issued behind a load into LDS waits
-                // for it), once this wave's reads of its rows are complete
-                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                acc_chirp_to_lds(Hd + (size_t)s * L + (size_t)r * S, rows, wv, lane);
-                if (s + 1 < Q)
-                    dl = acc_delays<R, Q>(a, d, s + 1, r);
-                else if (r + 1 < R)
-                    dl = acc_delays<R, Q>(a, d, 0, r + 1);
-                else if (next)
-                    dl = acc_delays<R, Q>(a, dn, 0, 0);
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const float4 h = hrow[64 * i];
-                    v[2 * i] = cadd(v[2 * i], cmul(w[2 * i], make_float2(h.x, h.y)));
-                    v[2 * i + 1] = cadd(v[2 * i + 1], cmul(w[2 * i + 1], make_float2(h.z, h.w)));
-                }
-            }
+            for (int t = 0; t < 16; ++t) v[t] = ld32(X, p0 + 1024u * t);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         // pass A: columns over k2, then W_L^{n2 (R k1 + r)} (the four-step twiddle times W_L^{r n2})
         subfft<ACC_N, 32, true, +1>(v, tile, col, jtA, tw.t128);
@@ -305,9 +256,11 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
             // no LDS. Without a branch on r (one costs the register allocator dearly) the last transform
             // requests its own chirp again; that request drains before the rows become the fold buffer.
             // The scheduling barrier keeps the requests ahead of the combine.
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            acc_chirp_to_lds(Hd + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
-            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (Q == 1) {
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                acc_chirp_to_lds(Hd + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
+                __builtin_amdgcn_sched_barrier(0);
+            }
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
 #pragma unroll
@@ -469,7 +422,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
     }
     __syncthreads();
     if (f16 && tid == 0) *pw_lds = 0.f;
-    if (next) acc_slot_table<R, Q>(a, bn, slot_next);
+    if constexpr (Q == 1)
+        if (next) acc_slot_table<R>(a, bn, slot_next);
     if (fold) fold_flush(a, d, hsum, hcnt, tid, WG);
     __syncthreads();
 }

@@ -416,10 +416,12 @@ def bench_uneven(args, nested: bool = False):
     mode, tms = multi.autotune(base, 3)
     mode_name = {ms.LAUNCH_PERSISTENT: "one launch over all classes", ms.LAUNCH_PER_CLASS: "one launch per class"}[mode]
     log(f"uneven autotune: {mode_name}, {tms:.3f} ms per push ({time.perf_counter() - t:.1f} s)")
+    # outside the timed region, on the tuned grid (autotune has reset it): the first push against the oracle
+    multi.set_launch_mode(mode)
+    rel = rel_err_uneven(ms, design, multi, data) if nested else None
     one = timed(multi, mode, pushes, args.steps, graph=args.graph and not nested and mode == ms.LAUNCH_PERSISTENT)
     log(f"uneven tuned: {one['rate']:.3e} DM-trial*samples/s, {one['ms_per_push']:.3f} ms/push")
     multi.close()
-    rel = rel_err_uneven(ms, design, W["push_blocks"], data) if nested else None
     # the plain form: one launch per DM, at the harness's 32 blocks per call and at other batches
     per_dm = {}
     for pb in (4, 32, 64):
@@ -466,13 +468,13 @@ def bench_uneven(args, nested: bool = False):
     print(json.dumps(out), flush=True)
 
 
-def rel_err_uneven(ms, design, push_blocks: int, data) -> dict:
-    """One trial of each length class against the FP64 oracle over the record's first push (zero history)."""
+def rel_err_uneven(ms, design, multi, data) -> dict:
+    """One trial of each length class against the FP64 oracle over the record's first push (zero history),
+    through the given grid in its current (autotuned, freshly reset) state, so in the timed launch forms."""
     import numpy as np
 
     from oracle import oracle as orc
 
-    multi = ms.Multi(design, push_blocks=push_blocks, in_format=ms.IN_C64, out_flags=ms.OUT_C64, device=0)
     per = multi.push_samples
     x = data[:per].cpu().numpy()
     multi.push(x)
@@ -484,7 +486,6 @@ def rel_err_uneven(ms, design, push_blocks: int, data) -> dict:
         ref = orc.msosm(x.astype(np.complex128), int(M), design.delays[d][None], design.H[d][None].astype(np.complex128))[0]
         l2, mx = rel_err_series(got, ref)
         res[str(int(2 * M))] = {"dm": float(design.dms[d]), "rel_l2": l2, "max_abs_rel": mx}
-    multi.close()
     return {"per_length": res, "rel_l2_max": max(v["rel_l2"] for v in res.values()), "samples": per, "bar": 1e-5}
 
 

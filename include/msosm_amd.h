@@ -184,9 +184,10 @@ enum {
      * unit's inverse transform is split by decimation in time into 1, 2 or 4 transforms of 2^14 points
      * that stay in the CU, the kept outputs accumulated in registers: no scratch slot round trip.
      * Lengths 2^17 and 2^18 (plans without FP16-pair output): each unit is split by decimation in
-     * frequency into L / 2^15 sub-units of that form, each keeping every (L / 2^15)-th output; a
-     * tangent push of such a plan runs the sequential form. The unit order does not apply to the
-     * sub-unit launch (it runs a DM's blocks and sub-units together on one XCD).
+     * frequency into L / 2^16 sub-units of that form, each keeping every (L / 2^16)-th output, after a
+     * gathering pass that writes the sub-units' spectra; a tangent push of such a plan runs the
+     * sequential form. The unit order does not apply to the sub-unit launch (it runs a DM's blocks
+     * and sub-units together on one XCD).
      * Takes precedence over MSOSM_SCHEDULE_SPLIT; other lengths run the sequential form */
     MSOSM_SCHEDULE_ACCUM = 16
 };

@@ -1,11 +1,11 @@
 """GPU parity of the sub-unit accumulate form (FFT lengths 2^17 and 2^18) against the FP64 oracle.
 
 At these lengths a unit's kept outputs do not fit a CU, so the accumulate launch splits each unit
-(DM, block) by decimation in frequency into Q = L / 2^15 sub-units: sub-unit c dedisperses the
-spectrum X_c[k'] = sum_s Y[k' + s 2^15] e^{+2 pi i c (k' + s 2^15) / L} with a 2^15-point accumulate
-transform and keeps the outputs y[M + Q u + c]. Another factorisation again, so its bar is the
-oracle's (rel-L2 1e-5, fold counts bit-exact), agreement with the two-pass form to FP32 rounding, and
-bit-identity with itself under every workgroup count and launch form.
+(DM, block) by decimation in frequency into Q = L / 2^16 sub-units: a gathering pass writes the
+spectra X_c[k'] = sum_s Y[k' + s 2^16] e^{+2 pi i c (k' + s 2^16) / L}, and sub-unit c dedisperses X_c
+with a 2^16-point accumulate transform, keeping the outputs y[M + Q u + c]. Another factorisation
+again, so its bar is the oracle's (rel-L2 1e-5, fold counts bit-exact), agreement with the two-pass
+form to FP32 rounding, and bit-identity with itself under every workgroup count and launch form.
 """
 import numpy as np
 import pytest
