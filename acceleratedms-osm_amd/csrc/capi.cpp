@@ -360,6 +360,7 @@ struct msosm_plan {
     // these offsets of tw (acc_q = 1: one unit, no gathering pass)
     int acc_q = 1, tw_sub_lo = 0, tw_sub_hi = 0, tw_sub_128 = 0, tw_sub_s = 0;
     float2 *xpre = nullptr;
+    unsigned int *sub_sync = nullptr; // [D][B] arrival counters of the sub-unit launch
     float2 *ring_p = nullptr, *H_p = nullptr, *Hdot_p = nullptr;
     uint8_t *delay_acc = nullptr;
     int sched_wg = 0, order = MSOSM_ORDER_XCD_DM_MAJOR; // tile schedule (0 workgroups = G)
@@ -385,7 +386,7 @@ static void plan_free(msosm_plan *p)
     if (p->out_stream) (void)hipStreamSynchronize(p->out_stream);
     void *bufs[] = {p->in_dev[0], p->in_dev[1], p->hist, p->ring, p->H, p->scratch, p->out_c64, p->tw, p->delay,
                     p->bins, p->out_u16b[0], p->out_u16b[1], p->out_f16, p->out_scale, p->Hdot, p->tan_y, p->ring_p,
-                    p->H_p, p->Hdot_p, p->delay_acc, p->xpre};
+                    p->H_p, p->Hdot_p, p->delay_acc, p->xpre, p->sub_sync};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : {p->in_ev[0], p->in_ev[1], p->in_done[0], p->in_done[1], p->out_ready[0], p->out_ready[1],
@@ -545,6 +546,7 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
     if (e == hipSuccess && p->accum_ok && p->acc_q > 1) {
         // the sub-unit spectra of one push: [D][B][acc_q][L / acc_q] complex64
         e = dalloc(p, &p->xpre, D * B * L * sizeof(float2));
+        if (e == hipSuccess) e = dalloc(p, &p->sub_sync, D * B * sizeof(unsigned int));
     } else if (e == hipSuccess && p->accum_ok) {
         e = dalloc(p, &p->delay_acc, D * L);
         if (e == hipSuccess) e = hipMemset(p->delay_acc, 0, D * L);
@@ -744,6 +746,7 @@ void msosm::plan_push_args(msosm_plan *p, const void *src, PushArgs *out, bool a
         ia.sub_128 = tw + p->tw_sub_128;
         ia.sub_s = p->tw_sub_s;
         ia.xpre = p->xpre;
+        ia.sub_sync = p->sub_sync;
     }
     if (p->fold) {
         msosm_fold *f = p->fold;
@@ -877,6 +880,7 @@ static int push_impl(msosm_plan *p, const void *input, int mode, msosm_fold *tfo
                 one.delay = x.delay + (size_t)d * p->L;
                 one.H_p = x.H_p + (size_t)d * p->L;
                 if (x.xpre) one.xpre = x.xpre + (size_t)d * p->B * p->L;
+                if (x.sub_sync) one.sub_sync = x.sub_sync + (size_t)d * p->B;
                 if (x.delay_acc) one.delay_acc = x.delay_acc + (size_t)d * p->L;
                 if (x.out_c64) one.out_c64 = x.out_c64 + d * ser;
                 if (x.out_u16) one.out_u16 = x.out_u16 + d * ser;

@@ -77,6 +77,10 @@ struct InvArgs {
     const float2 *sub_lo, *sub_hi, *sub_128;
     int sub_s;
     float2 *xpre;
+    // per unit (d B + b): arrival counters of its sub-units, zeroed by the gathering pass (or null). The Q
+    // sub-units of a unit wait for each other, up to a bounded number of polls, before writing their
+    // samples, so that every cache line of the unit's series is filled while it sits in one L2.
+    unsigned int *sub_sync;
 };
 
 // One persistent launch over several FFT lengths ("length classes"): class c owns the units

@@ -48,19 +48,32 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
 // at position perm_pos(k', 2, 14) of sub-unit c (the order in which acc_unit's transforms read them).
 // The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
 // factors) and its powers c = 2, 3 from products of it.
+// Workgroups take the DMs fastest (neighbouring workgroups gather the same bins of one ring for different
+// trials), and the spectra are written with non-temporal stores: they are read back by the next kernel,
+// long after any cache would have kept them, and the ring lines stay in the L2s instead.
+__device__ __forceinline__ void st_stream(float2 *p, float2 v)
+{
+    __builtin_nontemporal_store(__builtin_bit_cast(double, v), reinterpret_cast<double *>(p));
+}
+
+constexpr int PRE_WG = 256; // threads of a gathering-pass workgroup
+
 template <int Q>
-__global__ void __launch_bounds__(256) acc_pre_kernel(InvArgs a)
+__global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 {
     constexpr int LS = 1 << 16, CH = 4096;
     constexpr long long L = (long long)Q * LS;
-    const int chunk = (int)(blockIdx.x % (LS / CH)), d = (int)(blockIdx.x / (LS / CH));
+    const int d = (int)(blockIdx.x % a.D), chunk = (int)(blockIdx.x / a.D);
+    // the sub-unit launch's arrival counters of this DM's units start at zero
+    if (a.sub_sync && chunk == 0)
+        for (int b = (int)threadIdx.x; b < a.B; b += PRE_WG) a.sub_sync[(size_t)d * a.B + b] = 0u;
     const uint16_t *__restrict__ dly = a.delay + (size_t)d * L;
     const float2 *__restrict__ H = a.H + (size_t)d * L;
     float2 *__restrict__ X = a.xpre + (size_t)d * a.B * (size_t)L;
     const int mask = (1 << a.tw_s) - 1;
 #pragma unroll 1
-    for (int j = 0; j < CH / 256; ++j) {
-        const int kp = chunk * CH + j * 256 + (int)threadIdx.x;
+    for (int j = 0; j < CH / PRE_WG; ++j) {
+        const int kp = chunk * CH + j * PRE_WG + (int)threadIdx.x;
         int dd[Q];
         float2 h[Q];
 #pragma unroll
@@ -85,9 +98,9 @@ __global__ void __launch_bounds__(256) acc_pre_kernel(InvArgs a)
             }
             dft<+1, Q>(y);
             float2 *Xb = X + (size_t)b * L;
-            Xb[pos] = y[0];
+            st_stream(Xb + pos, y[0]);
 #pragma unroll
-            for (int c = 1; c < Q; ++c) Xb[(size_t)c * LS + pos] = cmul(y[c], wc[c]);
+            for (int c = 1; c < Q; ++c) st_stream(Xb + (size_t)c * LS + pos, cmul(y[c], wc[c]));
         }
     }
 }
@@ -119,6 +132,9 @@ __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
     uint4 dl = make_uint4(0u, 0u, 0u, 0u); // unused by sub-units
+    // the Q sub-units of a unit are consecutive items: with G a multiple of 8 Q they run in the same round
+    // of the walk on one XCD, and wait for each other before writing (otherwise they do not wait)
+    const bool sync = a.sub_sync != nullptr && G % (8 * Q) == 0;
 #pragma unroll 1
     while (have) {
         const long long xn = x + G;
@@ -126,7 +142,7 @@ __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
         int dn = 0, bn = 0, cn = 0;
         if (next) item(xn, dn, bn, cn);
         acc_unit<R, RING32, VAR_PLAIN, Q>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, nullptr,
-                                          nullptr);
+                                          nullptr, sync ? a.sub_sync + (size_t)d * a.B + b : nullptr);
         x = xn;
         d = dn;
         b = bn;
@@ -219,7 +235,7 @@ static hipError_t launch_acc_sub_t(const InvArgs &a, int grid, hipStream_t st)
     // the sub-unit form has the plain epilogue only; the caller chooses another form otherwise
     if (a.tangent_ref || a.out_f16 || !a.sub_lo || !a.xpre) return hipErrorInvalidValue;
     const unsigned pre_grid = (unsigned)((long long)a.D * ((1 << 16) / 4096));
-    hipLaunchKernelGGL((acc_pre_kernel<Q>), dim3(pre_grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((acc_pre_kernel<Q>), dim3(pre_grid), dim3(PRE_WG), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t lds = A::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0) + INV_EXTRA_LDS;

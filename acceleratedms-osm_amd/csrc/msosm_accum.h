@@ -126,11 +126,18 @@ struct AccR {
 //   X_c[k'] = sum_s Y[k' + s L'] e^{+2 pi i c (k' + s L') / L},   k' in [0, L'),
 // whose kept outputs [L'/2, L') are y[M + Q u + c], u in [0, L'/2) (M = L / 2). The gathering pass
 // (acc_pre_kernel) has written X_c in this unit's transform order (a.xpre), so each transform reads
-// its inputs contiguously: no delays, ring offsets or chirps here, and dl and slot are unused.
+// its inputs contiguously: no delays, ring offsets or chirps here, and dl and slot are unused. Sub-unit
+// c writes every Q-th sample of the block; with sync (the unit's arrival counter, zero before its first
+// sub-unit arrives) the workgroup waits, for at most ACC_SYNC_POLLS polls, until all Q sub-units of the
+// unit have finished their transforms, so that the Q workgroups (on one XCD) write each cache line of
+// the series at about the same time and the line leaves the L2 whole. The outputs do not depend on it.
+constexpr int ACC_SYNC_POLLS = 96;
+
 template <int R, bool RING32, int VAR, int Q = 1>
 __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, float2 *lds_all, const AccTw &tw,
                                          double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn,
-                                         int bn, const uint32_t *slot, uint32_t *slot_next)
+                                         int bn, const uint32_t *slot, uint32_t *slot_next,
+                                         unsigned int *sync = nullptr)
 {
     using A = AccShape<R>;
     constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
@@ -318,6 +325,15 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
 #pragma unroll 1
     for (int r = peel0 ? 1 : 0; r < R; ++r) transform(AccR<-1>{}, r);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); // the last chirp request has landed
+    if constexpr (Q > 1) {
+        if (sync && threadIdx.x == 0) {
+            __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int i = 0; i < ACC_SYNC_POLLS &&
+                            __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned int)Q;
+                 ++i)
+                __builtin_amdgcn_s_sleep(8);
+        }
+    }
     __syncthreads(); // every wave's pass B is done: the area becomes the fold buffer
 
     // outputs: element e of acc0 is sample m = j (R = 2, 4) or j - S/2 (R = 1), acc1 is m = j + S
