@@ -73,7 +73,7 @@ struct InvArgs {
     // sub-unit accumulate launch (lengths 2^17, 2^18: Q = L / 2^16 sub-units of 2^16 points per unit):
     // the twiddle tables of the sub-unit length (lo, hi split at sub_s; the 128-point roots), and the
     // sub-unit spectra [D][B][Q][2^16] that the gathering pass writes (sub-unit c of unit (d, b) at
-    // ((d B + b) Q + c) 2^16, in the accumulate layout perm_pos with R = 4)
+    // ((d B + b) Q + c) 2^16, bin k' at acc_chirp_pos(perm_pos(k', 2, 14)): the chirp table's order)
     const float2 *sub_lo, *sub_hi, *sub_128;
     int sub_s;
     float2 *xpre;

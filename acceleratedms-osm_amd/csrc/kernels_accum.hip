@@ -40,28 +40,35 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     }
 }
 
-// Gathering pass of the sub-unit accumulate launch (L = Q 2^16): one 256-thread workgroup per
-// (DM d, chunk of 4096 sub-spectrum bins), over every block b of the push. For each k' it reads the Q
-// bins k = k' + s 2^16 (their delays and chirps once, the ring slot of each block's delay: Y_b[k]), and
-// writes the Q sub-unit spectra of each block
+// Gathering pass of the sub-unit accumulate launch (L = Q 2^16): one workgroup per (DM d, chunk of 8192
+// sub-spectrum bins), over every block b of the push. For each k' it reads the Q bins k = k' + s 2^16
+// (their delays and chirps once, the ring slot of each block's delay: Y_b[k]), and writes the Q sub-unit
+// spectra of each block
 //   X_c[k'] = e^{+2 pi i c k' / L} sum_s e^{+2 pi i c s / Q} Y_b[k' + s 2^16]   (a Q-point DFT over s)
-// at position perm_pos(k', 2, 14) of sub-unit c (the order in which acc_unit's transforms read them).
+// at position acc_chirp_pos(perm_pos(k', 2, 14)) of sub-unit c: the order in which acc_unit's transforms
+// load them straight into LDS, as they load the chirp. The values at the positions t = 2i, 2i + 1 of one
+// 16-byte word are the bins k' and k' + 4096, which the two lanes of a lane pair hold: the lanes swap
+// values so that the even lane stores the words of the even sub-units and the odd lane those of the odd.
 // The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
 // factors) and its powers c = 2, 3 from products of it.
 // Workgroups take the DMs fastest (neighbouring workgroups gather the same bins of one ring for different
 // trials), and the spectra are written with non-temporal stores: they are read back by the next kernel,
 // long after any cache would have kept them, and the ring lines stay in the L2s instead.
-__device__ __forceinline__ void st_stream(float2 *p, float2 v)
+__device__ __forceinline__ void st_stream(float4 *p, float4 v)
 {
-    __builtin_nontemporal_store(__builtin_bit_cast(double, v), reinterpret_cast<double *>(p));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(__builtin_bit_cast(v4f, v), reinterpret_cast<v4f *>(p));
 }
 
-constexpr int PRE_WG = 256; // threads of a gathering-pass workgroup
+constexpr int PRE_WG = 256;           // threads of a gathering-pass workgroup
+constexpr int PRE_CH = 8192;          // sub-spectrum bins of a workgroup
+constexpr int PRE_CHUNKS = (1 << 16) / PRE_CH;
 
 template <int Q>
 __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 {
-    constexpr int LS = 1 << 16, CH = 4096;
+    static_assert(Q % 2 == 0, "sub-units come in pairs");
+    constexpr int LS = 1 << 16, HALF = PRE_CH / 2;
     constexpr long long L = (long long)Q * LS;
     const int d = (int)(blockIdx.x % a.D), chunk = (int)(blockIdx.x / a.D);
     // the sub-unit launch's arrival counters of this DM's units start at zero
@@ -71,9 +78,11 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
     const float2 *__restrict__ H = a.H + (size_t)d * L;
     float2 *__restrict__ X = a.xpre + (size_t)d * a.B * (size_t)L;
     const int mask = (1 << a.tw_s) - 1;
+    const int e = (int)threadIdx.x & 1, m = (int)threadIdx.x >> 1; // lane pair (bins k', k' + 4096)
 #pragma unroll 1
-    for (int j = 0; j < CH / PRE_WG; ++j) {
-        const int kp = chunk * CH + j * PRE_WG + (int)threadIdx.x;
+    for (int j = 0; j < HALF / (PRE_WG / 2); ++j) {
+        const int k0 = chunk * PRE_CH + j * (PRE_WG / 2) + m; // the pair's even bin: position t even
+        const int kp = k0 + e * HALF;
         int dd[Q];
         float2 h[Q];
 #pragma unroll
@@ -86,7 +95,7 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
         wc[0] = make_float2(1.f, 0.f);
 #pragma unroll
         for (int c = 1; c < Q; ++c) wc[c] = c == 1 ? w1 : cmul(wc[c - 1], w1);
-        const uint32_t pos = perm_pos((uint32_t)kp, 2, 14);
+        const uint32_t word = acc_chirp_pos(perm_pos((uint32_t)k0, 2, 14)); // even: its 16-byte word
 #pragma unroll 1
         for (int b = 0; b < a.B; ++b) {
             float2 y[Q];
@@ -97,10 +106,19 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
                 y[s] = cmul(a.ring[sl * L + kp + (long long)s * LS], h[s]);
             }
             dft<+1, Q>(y);
-            float2 *Xb = X + (size_t)b * L;
-            st_stream(Xb + pos, y[0]);
 #pragma unroll
-            for (int c = 1; c < Q; ++c) st_stream(Xb + (size_t)c * LS + pos, cmul(y[c], wc[c]));
+            for (int c = 1; c < Q; ++c) y[c] = cmul(y[c], wc[c]);
+            float2 *Xb = X + (size_t)b * L;
+#pragma unroll
+            for (int c2 = 0; c2 < Q; c2 += 2) {
+                // the even lane keeps sub-unit c2 and takes its partner's value of c2; the odd lane keeps
+                // c2 + 1 and takes its partner's value of c2 + 1
+                const float2 send = e ? y[c2] : y[c2 + 1];
+                const float2 got = make_float2(__shfl_xor(send.x, 1), __shfl_xor(send.y, 1));
+                const float2 lo = e ? got : y[c2], hi = e ? y[c2 + 1] : got;
+                st_stream(reinterpret_cast<float4 *>(Xb + (size_t)(c2 + e) * LS + word),
+                          make_float4(lo.x, lo.y, hi.x, hi.y));
+            }
         }
     }
 }
@@ -234,7 +252,7 @@ static hipError_t launch_acc_sub_t(const InvArgs &a, int grid, hipStream_t st)
     using A = AccShape<4>;
     // the sub-unit form has the plain epilogue only; the caller chooses another form otherwise
     if (a.tangent_ref || a.out_f16 || !a.sub_lo || !a.xpre) return hipErrorInvalidValue;
-    const unsigned pre_grid = (unsigned)((long long)a.D * ((1 << 16) / 4096));
+    const unsigned pre_grid = (unsigned)((long long)a.D * PRE_CHUNKS);
     hipLaunchKernelGGL((acc_pre_kernel<Q>), dim3(pre_grid), dim3(PRE_WG), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

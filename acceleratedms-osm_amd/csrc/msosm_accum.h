@@ -125,8 +125,8 @@ struct AccR {
 // sub-units c, each the unit above of length L' = L / Q on the spectrum
 //   X_c[k'] = sum_s Y[k' + s L'] e^{+2 pi i c (k' + s L') / L},   k' in [0, L'),
 // whose kept outputs [L'/2, L') are y[M + Q u + c], u in [0, L'/2) (M = L / 2). The gathering pass
-// (acc_pre_kernel) has written X_c in this unit's transform order (a.xpre), so each transform reads
-// its inputs contiguously: no delays, ring offsets or chirps here, and dl and slot are unused. Sub-unit
+// (acc_pre_kernel) has written X_c in the chirp's order (a.xpre), so each transform loads its inputs
+// into LDS as the chirp is loaded: no delays, ring offsets or chirps here, and dl and slot are unused. Sub-unit
 // c writes every Q-th sample of the block; with sync (the unit's arrival counter, zero before its first
 // sub-unit arrives) the workgroup waits, for at most ACC_SYNC_POLLS polls, until all Q sub-units of the
 // unit have finished their transforms, so that the Q workgroups (on one XCD) write each cache line of
@@ -148,6 +148,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
     const bool fold = a.fold_sum != nullptr;
     const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring_p;
+    // sub-units: this sub-unit's spectrum (transform r at Xs + r S, in the chirp's order)
+    const float2 *__restrict__ Xs = Q == 1 ? nullptr : a.xpre + (((size_t)d * a.B + b) * Q + c) * (size_t)L;
     float power = 0.f;
     // R = 4: transform 0 has its own code (its combine factors are 1: the accumulators start as its
     // outputs) and one loop runs r = 1..3; R = 1, 2: one loop from zeroed accumulators (a separate
@@ -206,12 +208,17 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
                 }
             }
         } else {
-            // the sub-unit spectrum from the gathering pass, already in this transform's order
-            const float2 *__restrict__ X = a.xpre + (((size_t)d * a.B + b) * Q + c) * (size_t)L;
-            const uint32_t p0 = (uint32_t)(r * S + k1 + 128 * jtA);
-#pragma unroll
-            for (int t = 0; t < 16; ++t) v[t] = ld32(X, p0 + 1024u * t);
+            // the sub-unit spectrum from the gathering pass, in the chirp's order: in the wave's LDS rows,
+            // put there during the previous transform's combine or, for r = 0, requested here
+            if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Xs, rows, wv, lane);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float4 h = hrow[64 * i];
+                v[2 * i] = make_float2(h.x, h.y);
+                v[2 * i + 1] = make_float2(h.z, h.w);
+            }
         }
         // pass A: columns over k2, then W_L^{n2 (R k1 + r)} (the four-step twiddle times W_L^{r n2})
         subfft<ACC_N, 32, true, +1>(v, tile, col, jtA, tw.t128);
@@ -258,14 +265,15 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
                 ct = acc_wl(tw, 2048 * r);
                 w = acc_wl(tw, 128 * r * jtB);
             }
-            // the next transform's chirp into this wave's rows: their pass-B exchange reads are complete,
+            // the next transform's chirp (sub-units: its spectrum) into this wave's rows: their pass-B
+            // exchange reads are complete,
             // no other wave touches them before the next pass A's first barrier, and the combine below reads
             // no LDS. Without a branch on r (one costs the register allocator dearly) the last transform
             // requests its own chirp again; that request drains before the rows become the fold buffer.
             // The scheduling barrier keeps the requests ahead of the combine.
-            if constexpr (Q == 1) {
+            {
                 asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                acc_chirp_to_lds(Hd + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
+                acc_chirp_to_lds((Q == 1 ? Hd : Xs) + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
