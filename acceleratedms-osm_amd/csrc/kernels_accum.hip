@@ -40,15 +40,19 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     }
 }
 
-// Gathering pass of the sub-unit accumulate launch (L = Q 2^16): one workgroup per (DM d, chunk of 8192
-// sub-spectrum bins), over every block b of the push. For each k' it reads the Q bins k = k' + s 2^16
-// (their delays and chirps once, the ring slot of each block's delay: Y_b[k]), and writes the Q sub-unit
-// spectra of each block
+// Gathering pass of the sub-unit accumulate launch (L = Q 2^16): one workgroup per (DM d, chunk i of 8192
+// sub-spectrum bins), over every block b of the push. For each bin k' it reads the Q bins
+// k = k' + s 2^16 (their delays and chirps once, the ring slot of each block's delay: Y_b[k]), and writes
+// the Q sub-unit spectra of each block
 //   X_c[k'] = e^{+2 pi i c k' / L} sum_s e^{+2 pi i c s / Q} Y_b[k' + s 2^16]   (a Q-point DFT over s)
 // at position acc_chirp_pos(perm_pos(k', 2, 14)) of sub-unit c: the order in which acc_unit's transforms
-// load them straight into LDS, as they load the chirp. The values at the positions t = 2i, 2i + 1 of one
-// 16-byte word are the bins k' and k' + 4096, which the two lanes of a lane pair hold: the lanes swap
-// values so that the even lane stores the words of the even sub-units and the odd lane those of the odd.
+// load them straight into LDS, as they load the chirp. In that order the chunk's bins fill, for each
+// sub-unit c and r = k' mod 4, the 16 rows (r, w) of 64 16-byte words (position
+// r 2^14 + 1024 w + 128 i + 2 l + e, e = 0 for bin k', 1 for k' + 4096). The workgroup takes one w at a
+// time: its 512 threads gather and transform one bin each (threads 256 e + 128 h + m: bin
+// k' = 8192 i + 4 base(w) + 512 h + m + 4096 e with base(w) = 32 (w div 4) + 256 (w mod 4), i.e. r = m mod 4,
+// word l = m div 4 + 32 h), put the values of each block into LDS in the rows' order, and each wave
+// stores whole 1 KiB rows.
 // The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
 // factors) and its powers c = 2, 3 from products of it.
 // Workgroups take the DMs fastest (neighbouring workgroups gather the same bins of one ring for different
@@ -60,42 +64,49 @@ __device__ __forceinline__ void st_stream(float4 *p, float4 v)
     __builtin_nontemporal_store(__builtin_bit_cast(v4f, v), reinterpret_cast<v4f *>(p));
 }
 
-constexpr int PRE_WG = 256;           // threads of a gathering-pass workgroup
-constexpr int PRE_CH = 8192;          // sub-spectrum bins of a workgroup
+constexpr int PRE_WG = 512;  // threads of a gathering-pass workgroup
+constexpr int PRE_CH = 8192; // sub-spectrum bins of a workgroup
 constexpr int PRE_CHUNKS = (1 << 16) / PRE_CH;
 
 template <int Q>
 __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 {
-    static_assert(Q % 2 == 0, "sub-units come in pairs");
-    constexpr int LS = 1 << 16, HALF = PRE_CH / 2;
+    static_assert(Q == 2 || Q == 4, "two or four sub-units");
+    constexpr int LS = 1 << 16, ROWS = 4 * Q; // rows (c, r) of one w
     constexpr long long L = (long long)Q * LS;
+    // two buffers of the rows (c, r) of one w: 64 words of 16 bytes each
+    __shared__ float4 rows_lds[2][ROWS][64];
     const int d = (int)(blockIdx.x % a.D), chunk = (int)(blockIdx.x / a.D);
+    const int tid = (int)threadIdx.x;
     // the sub-unit launch's arrival counters of this DM's units start at zero
     if (a.sub_sync && chunk == 0)
-        for (int b = (int)threadIdx.x; b < a.B; b += PRE_WG) a.sub_sync[(size_t)d * a.B + b] = 0u;
+        for (int b = tid; b < a.B; b += PRE_WG) a.sub_sync[(size_t)d * a.B + b] = 0u;
     const uint16_t *__restrict__ dly = a.delay + (size_t)d * L;
     const float2 *__restrict__ H = a.H + (size_t)d * L;
     float2 *__restrict__ X = a.xpre + (size_t)d * a.B * (size_t)L;
     const int mask = (1 << a.tw_s) - 1;
-    const int e = (int)threadIdx.x & 1, m = (int)threadIdx.x >> 1; // lane pair (bins k', k' + 4096)
+    const int e = tid >> 8, h = (tid >> 7) & 1, m = tid & 127;
+    const int r = m & 3, l = (m >> 2) + 32 * h;
+    float *slot = reinterpret_cast<float *>(&rows_lds[0][r][l]) + 2 * e; // this thread's half word, c = 0
+    const int wave = tid >> 6, lane = tid & 63;
+    int buf = 0;
 #pragma unroll 1
-    for (int j = 0; j < HALF / (PRE_WG / 2); ++j) {
-        const int k0 = chunk * PRE_CH + j * (PRE_WG / 2) + m; // the pair's even bin: position t even
-        const int kp = k0 + e * HALF;
+    for (int w = 0; w < 16; ++w) {
+        const int kp = chunk * PRE_CH + 4 * (32 * (w >> 2) + 256 * (w & 3)) + 512 * h + m + 4096 * e;
         int dd[Q];
-        float2 h[Q];
+        float2 hc[Q];
 #pragma unroll
         for (int s = 0; s < Q; ++s) {
             dd[s] = dly[kp + (long long)s * LS];
-            h[s] = H[kp + (long long)s * LS];
+            hc[s] = H[kp + (long long)s * LS];
         }
         const float2 w1 = cmul(a.tw_hi[kp >> a.tw_s], a.tw_lo[kp & mask]);
         float2 wc[Q];
         wc[0] = make_float2(1.f, 0.f);
 #pragma unroll
         for (int c = 1; c < Q; ++c) wc[c] = c == 1 ? w1 : cmul(wc[c - 1], w1);
-        const uint32_t word = acc_chirp_pos(perm_pos((uint32_t)k0, 2, 14)); // even: its 16-byte word
+        // row (c, rr) of this w, word lane: position c 2^16 + rr 2^14 + 1024 w + 128 chunk + 2 lane
+        const size_t row0 = (size_t)1024 * w + 128 * chunk + 2 * lane;
 #pragma unroll 1
         for (int b = 0; b < a.B; ++b) {
             float2 y[Q];
@@ -103,22 +114,25 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
             for (int s = 0; s < Q; ++s) {
                 long long sl = (a.g0 + b - (long long)dd[s]) % a.FBS;
                 if (sl < 0) sl += a.FBS;
-                y[s] = cmul(a.ring[sl * L + kp + (long long)s * LS], h[s]);
+                y[s] = cmul(a.ring[sl * L + kp + (long long)s * LS], hc[s]);
             }
             dft<+1, Q>(y);
+            float *sb = slot + buf * (ROWS * 64 * 4);
 #pragma unroll
-            for (int c = 1; c < Q; ++c) y[c] = cmul(y[c], wc[c]);
+            for (int c = 0; c < Q; ++c) {
+                const float2 v = c == 0 ? y[0] : cmul(y[c], wc[c]);
+                sb[c * 4 * 64 * 4] = v.x; // rows_lds[buf][4 c + r][l], half e
+                sb[c * 4 * 64 * 4 + 1] = v.y;
+            }
+            __syncthreads(); // the rows of this block are in LDS; the other buffer's readers are done
             float2 *Xb = X + (size_t)b * L;
 #pragma unroll
-            for (int c2 = 0; c2 < Q; c2 += 2) {
-                // the even lane keeps sub-unit c2 and takes its partner's value of c2; the odd lane keeps
-                // c2 + 1 and takes its partner's value of c2 + 1
-                const float2 send = e ? y[c2] : y[c2 + 1];
-                const float2 got = make_float2(__shfl_xor(send.x, 1), __shfl_xor(send.y, 1));
-                const float2 lo = e ? got : y[c2], hi = e ? y[c2 + 1] : got;
-                st_stream(reinterpret_cast<float4 *>(Xb + (size_t)(c2 + e) * LS + word),
-                          make_float4(lo.x, lo.y, hi.x, hi.y));
+            for (int k = 0; k < ROWS / 8; ++k) {
+                const int row = wave + 8 * k; // 4 c + rr
+                st_stream(reinterpret_cast<float4 *>(Xb + (size_t)(row >> 2) * LS + (size_t)(row & 3) * (1 << 14) + row0),
+                          rows_lds[buf][row][lane]);
             }
+            buf ^= 1;
         }
     }
 }
