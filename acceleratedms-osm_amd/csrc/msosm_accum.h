@@ -89,13 +89,17 @@ __device__ __forceinline__ float2 *acc_wave_rows(float2 *lds_all, int w)
 // Chirp values of transform r for wave w (acc_chirp_pos) from global memory straight into LDS: eight
 // 16-byte loads per lane, lane l of load i landing at rows + 128 i + 2 l; complete once the wave's
 // vector memory counter drains
-__device__ __forceinline__ void acc_chirp_to_lds(const float2 *Hr, float2 *rows, int w, int lane)
+__device__ __forceinline__ void acc_rows_from(const float2 *src, int step, float2 *rows)
 {
-    const float2 *src = Hr + 1024 * w + 2 * lane;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + 128 * i),
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + step * i),
                                          (__attribute__((address_space(3))) void *)(rows + 128 * i), 16, 0, 0);
+}
+
+__device__ __forceinline__ void acc_chirp_to_lds(const float2 *Hr, float2 *rows, int w, int lane)
+{
+    acc_rows_from(Hr + 1024 * w + 2 * lane, 128, rows);
 }
 
 // The 16 byte delays of this thread for transform r of DM d: the bytes at r S + 16 p (p = k1 + 128 jtA),
@@ -271,9 +275,14 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
             // no LDS. Without a branch on r (one costs the register allocator dearly) the last transform
             // requests its own chirp again; that request drains before the rows become the fold buffer.
             // The scheduling barrier keeps the requests ahead of the combine.
+            // Sub-units have nothing to request after the last transform (their spectra are read once): that
+            // request reads one 16-byte word into every row word instead (the rows are not read again
+            // before the next transform's own request), where a repeated spectrum would miss the L2s.
             {
                 asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                acc_chirp_to_lds((Q == 1 ? Hd : Xs) + (size_t)(r + 1 < R ? r + 1 : r) * S, rows, wv, lane);
+                const float2 *src = (Q == 1 ? Hd : Xs) + (size_t)(r + 1 < R ? r + 1 : r) * S;
+                const bool real = Q == 1 || r + 1 < R;
+                acc_rows_from(real ? src + 1024 * wv + 2 * lane : src, real ? 128 : 0, rows);
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
