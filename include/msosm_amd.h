@@ -187,7 +187,9 @@ enum {
      * frequency into L / 2^16 sub-units of that form, each keeping every (L / 2^16)-th output, after a
      * gathering pass that writes the sub-units' spectra; a tangent push of such a plan runs the
      * sequential form. The unit order does not apply to the sub-unit launch (it runs a DM's blocks
-     * and sub-units together on one XCD).
+     * and sub-units together on one XCD). Such plans hold the sub-unit spectra of one push in device
+     * memory from creation: 8 * num_dms * batch * L bytes (17 GB for 507 trials x 16 blocks at
+     * L = 2^18), and 4 * num_dms * batch bytes of arrival counters.
      * Takes precedence over MSOSM_SCHEDULE_SPLIT; other lengths run the sequential form */
     MSOSM_SCHEDULE_ACCUM = 16
 };
