@@ -384,7 +384,12 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
 #pragma unroll
         for (int i = 0; i < MPT / 8; ++i) bw[i] = bins4[i];
     }
-    auto out1 = [&](int m, float2 y) {
+    // fold buffer position of sample m = n2B + 128 n1 + e (e = 0, S or -S/2, all multiples of MPT, as 128
+    // is): (m mod MPT) FOLD_X + m div MPT = fb0 + (128 / MPT) n1 + e / MPT, a per-thread base and an
+    // offset known at compile time
+    static_assert(128 % MPT == 0 && (S / 2) % MPT == 0, "fold buffer positions split");
+    const int fb0 = (n2B % MPT) * A::FOLD_X + n2B / MPT;
+    auto out1 = [&](int m, int fbi, float2 y) {
         const uint32_t mo = Q == 1 ? (uint32_t)m : (uint32_t)(Q * m + c); // sample index in the block
         if (oc) st32(oc, mo, y);
         if (ou) st32(ou, mo, (enc_u16(y.y) << 16) | enc_u16(y.x));
@@ -398,19 +403,20 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
                 const float I = rf.x * rf.x + rf.y * rf.y;
                 v = I != 0.0f ? 2.f * (rf.x * y.x + rf.y * y.y) : __builtin_nanf("");
             }
-            fb[(m % MPT) * A::FOLD_X + m / MPT] = v;
+            fb[fbi] = v;
         }
     };
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int j = n2B + 128 * (jtB + 8 * q + 16 * t);
+            const int n1 = jtB + 8 * q + 16 * t, j = n2B + 128 * n1;
+            const int fbi = fb0 + (128 / MPT) * jtB + (128 / MPT) * (8 * q + 16 * t);
             if constexpr (R == 1) {
-                if (t >= 4) out1(j - S / 2, acc0[q * 4 + t - 4]);
+                if (t >= 4) out1(j - S / 2, fbi - (S / 2) / MPT, acc0[q * 4 + t - 4]);
             } else {
-                out1(j, acc0[q * 8 + t]);
-                if constexpr (R == 4) out1(j + S, acc1[q * 8 + t]);
+                out1(j, fbi, acc0[q * 8 + t]);
+                if constexpr (R == 4) out1(j + S, fbi + S / MPT, acc1[q * 8 + t]);
             }
         }
     if (fold) {
