@@ -146,3 +146,29 @@ def test_sub_accum_schedule_falls_back_for_tangent_pushes():
     for (pa, ta), (pb, tb) in [(res[0], res[1])]:
         for x, y in zip(pa + ta, pb + tb):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+@pytest.mark.parametrize("batch", [1, 3])
+def test_sub_accum_odd_batches_match_oracle_and_launch_forms(log2L, batch):
+    # one and three blocks per push: the gathering pass's double-buffered rows end on either buffer, and
+    # at Q = 2 its one-block-ahead ring request repeats the last block; the series against the oracle,
+    # and the same bits from the persistent launch (sub-units meeting before their writes) and the
+    # one-launch-per-DM form (grids that are no multiple of 8 Q do not meet)
+    design = _design(log2L)
+    calls = 3
+    per = batch * design.M
+    n = per * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, float(design.dms[1]), 0.0011,
+                        seed=120 + log2L + batch)
+    raw, x = np.ascontiguousarray(raw[:n]), x[:n]
+    print(f"case sub-unit accumulate L=2^{log2L} batch={batch} calls={calls}", flush=True)
+    got = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64)
+    per_dm = _run(design, raw, ms.IN_C64, batch, ms.OUT_C64, mode=ms.LAUNCH_PER_DM)
+    assert np.array_equal(got, per_dm)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    for d in range(len(design.dms)):
+        err = rel_l2(got[d], ref[d])
+        print(f"  DM {design.dms[d]}: rel-L2 {err:.3e}", flush=True)
+        assert err < TOL
