@@ -221,16 +221,29 @@ __device__ __forceinline__ void butterflies(float2 (&v)[16], int jt, const float
     }
 }
 
+// Workgroup barrier that orders LDS accesses only: the fences cover the LDS address space, so the
+// wave waits for its LDS operations but not for its global stores (which a full workgroup fence would
+// make it wait for). For exchanges through LDS whose threads share nothing through global memory.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Barrier of an LDS exchange. WAVE: every point a wave reads was written by the same wave (its
 // columns' regions are its own), and LDS instructions of one wave execute in order, so a
-// wavefront-scope fence that keeps the compiler from moving accesses across it suffices.
-template <bool WAVE>
+// wavefront-scope fence that keeps the compiler from moving accesses across it suffices. LDSB: a
+// workgroup barrier over LDS only (lds_barrier).
+template <bool WAVE, bool LDSB = false>
 __device__ __forceinline__ void exchange_barrier()
 {
     if constexpr (WAVE) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else if constexpr (LDSB) {
+        lds_barrier();
     } else {
         __syncthreads();
     }
@@ -256,7 +269,7 @@ __device__ __forceinline__ int in_pos(int jt, int q, int t)
 
 // Remaining passes after the one that ended at span Ns_done (radix R_done).
 // On return, v holds the final outputs; element (q, t) is output index out_pos<NS, RL, NsL>.
-template <int NS, int NCOL, bool COLFAST, int DIR, int R_done, int Ns_done, bool WAVE = false>
+template <int NS, int NCOL, bool COLFAST, int DIR, int R_done, int Ns_done, bool WAVE = false, bool LDSB = false>
 __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int col, int jt,
                                               const float2 *__restrict__ tw)
 {
@@ -264,7 +277,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
     if constexpr (Ns < NS) {
         constexpr int R = PassRadix<NS, Ns>::R;
         constexpr int TPC = NS / 16;
-        exchange_barrier<WAVE>(); // LDS free (previous readers done)
+        exchange_barrier<WAVE, LDSB>(); // LDS free (previous readers done)
         // output position (j - k) R + k + t Ns, j = jt + q TPC, k = j mod Ns: split into a per-thread
         // part and a compile-time part without carries when k does not depend on q
         constexpr bool out_split = (Ns_done == 1 && R_done == 16) || (Ns_done % 16 == 0 && TPC % Ns_done == 0);
@@ -284,7 +297,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
                 for (int t = 0; t < R_done; ++t)
                     lds[lds_index<NS, NCOL, COLFAST>(out_pos<NS, R_done, Ns_done>(jt, q, t), col)] = v[q * R_done + t];
         }
-        exchange_barrier<WAVE>();
+        exchange_barrier<WAVE, LDSB>();
         // input position jt + q TPC + t NS/R
         constexpr bool in_split = TPC % 16 == 0 && (NS / R) % 16 == 0;
         if constexpr (in_split) {
@@ -302,7 +315,7 @@ __device__ __forceinline__ void stockham_rest(float2 (&v)[16], float2 *lds, int 
                     v[q * R + t] = lds[lds_index<NS, NCOL, COLFAST>(in_pos<NS, R>(jt, q, t), col)];
         }
         butterflies<NS, R, Ns, DIR>(v, jt, tw);
-        stockham_rest<NS, NCOL, COLFAST, DIR, R, Ns, WAVE>(v, lds, col, jt, tw);
+        stockham_rest<NS, NCOL, COLFAST, DIR, R, Ns, WAVE, LDSB>(v, lds, col, jt, tw);
     }
 }
 
@@ -320,12 +333,13 @@ struct LastPass<NS, NS> {
 
 // Full sub-FFT. On entry v[t] = x[jt + t * NS/16] (t in [0,16)). On exit element (q, t) of v is
 // output index final_pos<NS>(jt, q, t). WAVE: the NS/16 threads of every column are lanes of one
-// wave and the caller has made the columns' LDS regions free (exchange_barrier).
-template <int NS, int NCOL, bool COLFAST, int DIR, bool WAVE = false>
+// wave and the caller has made the columns' LDS regions free (exchange_barrier). LDSB: see
+// exchange_barrier.
+template <int NS, int NCOL, bool COLFAST, int DIR, bool WAVE = false, bool LDSB = false>
 __device__ __forceinline__ void subfft(float2 (&v)[16], float2 *lds, int col, int jt, const float2 *__restrict__ tw)
 {
     butterflies<NS, 16, 1, DIR>(v, jt, tw);
-    stockham_rest<NS, NCOL, COLFAST, DIR, 16, 1, WAVE>(v, lds, col, jt, tw);
+    stockham_rest<NS, NCOL, COLFAST, DIR, 16, 1, WAVE, LDSB>(v, lds, col, jt, tw);
 }
 
 template <int NS>

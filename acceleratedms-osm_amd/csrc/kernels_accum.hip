@@ -27,12 +27,13 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     // during the previous one
     uint4 dl = have ? acc_delays<R>(a, d, 0) : make_uint4(0u, 0u, 0u, 0u);
     int cur = 0;
+    float2 xv[16]; // unused by one-unit launches
 #pragma unroll 1
     while (have) {
         int dn = 0, bn = 0;
         const bool next = o.next(s, a, dn, bn);
         acc_unit<R, RING32, VAR>(a, d, b, 0, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, slots + 256 * cur,
-                                 slots + 256 * (cur ^ 1));
+                                 slots + 256 * (cur ^ 1), nullptr, nullptr, xv);
         cur ^= 1;
         d = dn;
         b = bn;
@@ -159,20 +160,23 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 // Sub-unit accumulate kernel (L = Q 2^16, plain epilogue: complex64 / uint16 outputs, fused fold) on the
 // gathering pass's spectra. Items x = (d B + b) Q + c in the XCD-grouped order: the G/8 workgroups of an
 // XCD take consecutive items.
-template <int Q, bool RING32>
+template <int Q, bool RING32, int OUTK>
 __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
 {
+    constexpr bool ONE_OUT = OUTK != ACC_OUT_ANY;
     constexpr int R = 4;
     using A = AccShape<R>;
     extern __shared__ float2 lds_all[];
-    const long long total = (long long)a.D * a.B * Q;
+    const int total = __builtin_amdgcn_readfirstlane(a.D * a.B * Q);
     const int G = gridDim.x, w = blockIdx.x;
-    long long x = G % 8 == 0 ? (long long)(w % 8) * (G / 8) + w / 8 : w;
-    auto item = [&](long long v, int &d, int &b, int &c) {
-        c = (int)(v % Q);
-        const long long u = v / Q;
-        b = (int)(u % a.B);
-        d = (int)(u / a.B);
+    // item indices in 32 bits (the launch checks D B Q < 2^31) and in scalar registers: a wide or vector
+    // loop counter ends up in a spilled register whose reload would wait for the output stores
+    int x = G % 8 == 0 ? (w % 8) * (G / 8) + w / 8 : w;
+    auto item = [&](int v, int &d, int &b, int &c) {
+        c = __builtin_amdgcn_readfirstlane(v % Q);
+        const int u = v / Q;
+        b = __builtin_amdgcn_readfirstlane(u % a.B);
+        d = __builtin_amdgcn_readfirstlane(u / a.B);
     };
     int d = 0, b = 0, c = 0;
     bool have = x < total; // uniform over the workgroup
@@ -183,17 +187,26 @@ __global__ void __launch_bounds__(WG) acc_sub_kernel(InvArgs a)
     float *pw_lds = reinterpret_cast<float *>(hcnt + a.nbins + (a.nbins & 1));
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
     uint4 dl = make_uint4(0u, 0u, 0u, 0u); // unused by sub-units
+    float2 xv[16]; // one output: the item's first spectrum, loaded by the item before it
+    if constexpr (ONE_OUT) {
+        if (have) acc_x0_load(a.xpre + (((size_t)d * a.B + b) * Q + c) * (size_t)(1 << 16), xv);
+        // wait for them here, so that at the loop head only the previous item's stores follow the loads
+#pragma unroll
+        for (int t = 0; t < 16; ++t) asm volatile("" : "+v"(xv[t].x), "+v"(xv[t].y));
+    }
     // the Q sub-units of a unit are consecutive items: with G a multiple of 8 Q they run in the same round
     // of the walk on one XCD, and wait for each other before writing (otherwise they do not wait)
     const bool sync = a.sub_sync != nullptr && G % (8 * Q) == 0;
 #pragma unroll 1
     while (have) {
-        const long long xn = x + G;
+        const int xn = x + G;
         const bool next = xn < total;
         int dn = 0, bn = 0, cn = 0;
         if (next) item(xn, dn, bn, cn);
-        acc_unit<R, RING32, VAR_PLAIN, Q>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn, nullptr,
-                                          nullptr, sync ? a.sub_sync + (size_t)d * a.B + b : nullptr);
+        const float2 *xnext = ONE_OUT && next ? a.xpre + (((size_t)dn * a.B + bn) * Q + cn) * (size_t)(1 << 16) : nullptr;
+        acc_unit<R, RING32, VAR_PLAIN, Q, OUTK>(a, d, b, c, lds_all, tw, hsum, hcnt, pw_lds, dl, next, dn, bn,
+                                                   nullptr, nullptr, sync ? a.sub_sync + (size_t)d * a.B + b : nullptr,
+                                                   xnext, xv);
         x = xn;
         d = dn;
         b = bn;
@@ -238,7 +251,9 @@ static hipError_t configure_acc_t(int bytes)
 template <int Q>
 static hipError_t configure_acc_sub_t(int bytes)
 {
-    const void *fs[] = {reinterpret_cast<const void *>(&acc_sub_kernel<Q, true>)};
+    const void *fs[] = {reinterpret_cast<const void *>(&acc_sub_kernel<Q, true, ACC_OUT_ANY>),
+                        reinterpret_cast<const void *>(&acc_sub_kernel<Q, true, ACC_OUT_C64>),
+                        reinterpret_cast<const void *>(&acc_sub_kernel<Q, true, ACC_OUT_U16>)};
     for (const void *f : fs) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
@@ -285,12 +300,19 @@ static hipError_t launch_acc_sub_t(const InvArgs &a, int grid, hipStream_t st)
     using A = AccShape<4>;
     // the sub-unit form has the plain epilogue only; the caller chooses another form otherwise
     if (a.tangent_ref || a.out_f16 || !a.sub_lo || !a.xpre) return hipErrorInvalidValue;
+    if ((long long)a.D * a.B * Q >= (1ll << 31)) return hipErrorInvalidValue; // 32-bit item indices
     const unsigned pre_grid = (unsigned)((long long)a.D * PRE_CHUNKS);
     hipLaunchKernelGGL((acc_pre_kernel<Q>), dim3(pre_grid), dim3(PRE_WG), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t lds = A::LDS_BYTES + (a.fold_sum ? (size_t)a.nbins * 12 : 0) + INV_EXTRA_LDS;
-    hipLaunchKernelGGL((acc_sub_kernel<Q, true>), dim3(grid), dim3(WG), lds, st, a);
+    // exactly one output and no fold: the sub-units load each next spectrum ahead of their stores
+    if (!a.fold_sum && a.out_c64 && !a.out_u16)
+        hipLaunchKernelGGL((acc_sub_kernel<Q, true, ACC_OUT_C64>), dim3(grid), dim3(WG), lds, st, a);
+    else if (!a.fold_sum && !a.out_c64 && a.out_u16)
+        hipLaunchKernelGGL((acc_sub_kernel<Q, true, ACC_OUT_U16>), dim3(grid), dim3(WG), lds, st, a);
+    else
+        hipLaunchKernelGGL((acc_sub_kernel<Q, true, ACC_OUT_ANY>), dim3(grid), dim3(WG), lds, st, a);
     return hipGetLastError();
 }
 

@@ -135,21 +135,56 @@ struct AccR {
 // sub-unit arrives) the workgroup waits, for at most ACC_SYNC_POLLS polls, until all Q sub-units of the
 // unit have finished their transforms, so that the Q workgroups (on one XCD) write each cache line of
 // the series at about the same time and the line leaves the L2 whole. The outputs do not depend on it.
+// OUTK = ACC_OUT_C64 or ACC_OUT_U16 (sub-units with that one output and no fold, ONE_OUT): the item's first
+// spectrum arrives in registers (xv, loaded by the previous item or the kernel), and the next item's
+// (xn, when there is one) is loaded into xv just before this item's output stores, so that the next
+// item's first transform waits for those loads only and not for the stores (a wave's loads and stores
+// share its memory counter, which the compiler's waits count precisely for register loads); the
+// unit's barriers order LDS only, so the stores stay in flight through that transform.
 constexpr int ACC_SYNC_POLLS = 96;
+enum { ACC_OUT_ANY = 0, ACC_OUT_C64 = 1, ACC_OUT_U16 = 2 }; // outputs of a launch known at compile time
 
-template <int R, bool RING32, int VAR, int Q = 1>
+// The unit's workgroup barriers guard LDS only. LDSB: barriers that do not wait for the wave's global
+// stores, so that one item's output stores stay in flight through the next item's first transform.
+template <bool LDSB>
+__device__ __forceinline__ void acc_barrier()
+{
+    if constexpr (LDSB)
+        lds_barrier();
+    else
+        __syncthreads();
+}
+
+// The first spectrum of a sub-unit in registers, in the order of acc_unit's first transform: the row
+// words of wave w, lane l (positions 1024 w + 128 i + 2 l and + 1)
+__device__ __forceinline__ void acc_x0_load(const float2 *X, float2 (&xv)[16])
+{
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const float4 *src = reinterpret_cast<const float4 *>(X + 1024 * (tid >> 6) + 2 * (tid & 63));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float4 h = src[64 * i];
+        xv[2 * i] = make_float2(h.x, h.y);
+        xv[2 * i + 1] = make_float2(h.z, h.w);
+    }
+}
+
+template <int R, bool RING32, int VAR, int Q = 1, int OUTK = ACC_OUT_ANY>
 __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, float2 *lds_all, const AccTw &tw,
                                          double *hsum, unsigned int *hcnt, float *pw_lds, uint4 &dl, bool next, int dn,
                                          int bn, const uint32_t *slot, uint32_t *slot_next,
-                                         unsigned int *sync = nullptr)
+                                         unsigned int *sync, const float2 *xn, float2 (&xv)[16])
 {
+    constexpr bool ONE_OUT = OUTK != ACC_OUT_ANY;
+    static_assert(!ONE_OUT || (Q > 1 && R == 4), "one-output sub-units");
     using A = AccShape<R>;
     constexpr int L = A::L, M = A::M, S = ACC_S, MPT = A::MPT;
     constexpr bool f16 = (VAR & VAR_F16) != 0;
     constexpr bool tangent = (VAR & VAR_TANGENT) != 0;
     static_assert(Q == 1 || (R == 4 && !f16 && !tangent), "sub-units: R = 4, plain, uint16 or fold epilogues");
     constexpr int MB = Q * M; // samples per block
-    const bool fold = a.fold_sum != nullptr;
+    const bool fold = !ONE_OUT && a.fold_sum != nullptr;
     const float2 *__restrict__ Hd = a.H_p + (size_t)d * L;
     const float2 *__restrict__ ring = a.ring_p;
     // sub-units: this sub-unit's spectrum (transform r at Xs + r S, in the chirp's order)
@@ -213,19 +248,25 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
             }
         } else {
             // the sub-unit spectrum from the gathering pass, in the chirp's order: in the wave's LDS rows,
-            // put there during the previous transform's combine or, for r = 0, requested here
-            if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Xs, rows, wv, lane);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
+            // put there during the previous transform's combine or, for r = 0, requested here (one
+            // output: in registers already)
+            if constexpr (ONE_OUT && RC == 0) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const float4 h = hrow[64 * i];
-                v[2 * i] = make_float2(h.x, h.y);
-                v[2 * i + 1] = make_float2(h.z, h.w);
+                for (int t = 0; t < 16; ++t) v[t] = xv[t];
+            } else {
+                if (RC == 0 || (RC < 0 && r == 0)) acc_chirp_to_lds(Xs, rows, wv, lane);
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                const float4 *hrow = reinterpret_cast<const float4 *>(rows) + lane;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float4 h = hrow[64 * i];
+                    v[2 * i] = make_float2(h.x, h.y);
+                    v[2 * i + 1] = make_float2(h.z, h.w);
+                }
             }
         }
         // pass A: columns over k2, then W_L^{n2 (R k1 + r)} (the four-step twiddle times W_L^{r n2})
-        subfft<ACC_N, 32, true, +1>(v, tile, col, jtA, tw.t128);
+        subfft<ACC_N, 32, true, +1, false, ONE_OUT>(v, tile, col, jtA, tw.t128);
         {
             const int kk = R * k1 + r;
             const float2 c = acc_wl(tw, 16 * kk); // n2 steps by 16 along t
@@ -239,15 +280,15 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
                 }
             }
         }
-        __syncthreads(); // every tile's exchange reads are done: the area becomes the transpose
+        acc_barrier<ONE_OUT>(); // every tile's exchange reads are done: the area becomes the transpose
 #pragma unroll
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int t = 0; t < 8; ++t) lds_all[(jtA + 8 * q + 16 * t) * ACC_XS + k1] = v[q * 8 + t];
-        __syncthreads();
+        acc_barrier<ONE_OUT>();
 #pragma unroll
         for (int t = 0; t < 16; ++t) v[t] = lds_all[n2B * ACC_XS + jtB + 8 * t];
-        __syncthreads(); // the transpose is read: the area becomes the tile exchanges again
+        acc_barrier<ONE_OUT>(); // the transpose is read: the area becomes the tile exchanges again
         // pass B: rows over k1; element (q, t) is n1 = jtB + 8 q + 16 t, j = n2B + 128 n1. The 8 threads
         // of a row are consecutive lanes and a row's exchange region (row-major, 136 words) is its own,
         // so the exchanges need no workgroup barrier
@@ -352,6 +393,11 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
         }
     }
     __syncthreads(); // every wave's pass B is done: the area becomes the fold buffer
+    if constexpr (ONE_OUT) {
+        // the next item's first spectrum, ahead of the output stores (after the last item, this item's
+        // again: an unconditional load leaves xv dead through the transforms)
+        acc_x0_load(xn ? xn : Xs, xv);
+    }
 
     // outputs: element e of acc0 is sample m = j (R = 2, 4) or j - S/2 (R = 1), acc1 is m = j + S
     int tid = threadIdx.x;
@@ -370,8 +416,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
         inv_scale = 1.f / out_scale;
         if (tid == 0 && a.out_scale) a.out_scale[(size_t)d * a.B + b] = out_scale;
     }
-    float2 *oc = a.out_c64 ? a.out_c64 + ((size_t)d * a.B + b) * MB : nullptr;
-    uint32_t *ou = a.out_u16 ? a.out_u16 + ((size_t)d * a.B + b) * MB : nullptr;
+    float2 *oc = OUTK == ACC_OUT_C64 || (OUTK == ACC_OUT_ANY && a.out_c64) ? a.out_c64 + ((size_t)d * a.B + b) * MB : nullptr;
+    uint32_t *ou = OUTK == ACC_OUT_U16 || (OUTK == ACC_OUT_ANY && a.out_u16) ? a.out_u16 + ((size_t)d * a.B + b) * MB : nullptr;
     uint32_t *oh = f16 && a.out_f16 ? a.out_f16 + ((size_t)d * a.B + b) * MB : nullptr;
     const float2 *tref = tangent ? a.tangent_ref + ((size_t)d * a.B + b) * MB : nullptr;
     float *fb = reinterpret_cast<float *>(lds_all); // fold buffer: sample m at (m % MPT) FOLD_X + m / MPT
@@ -391,8 +437,8 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
     const int fb0 = (n2B % MPT) * A::FOLD_X + n2B / MPT;
     auto out1 = [&](int m, int fbi, float2 y) {
         const uint32_t mo = Q == 1 ? (uint32_t)m : (uint32_t)(Q * m + c); // sample index in the block
-        if (oc) st32(oc, mo, y);
-        if (ou) st32(ou, mo, (enc_u16(y.y) << 16) | enc_u16(y.x));
+        if (OUTK == ACC_OUT_C64 || (OUTK == ACC_OUT_ANY && oc)) st32(oc, mo, y);
+        if (OUTK == ACC_OUT_U16 || (OUTK == ACC_OUT_ANY && ou)) st32(ou, mo, (enc_u16(y.y) << 16) | enc_u16(y.x));
         if constexpr (f16)
             if (oh) st32(oh, mo, enc_f16_pair(y, inv_scale));
         if (fold) {
@@ -459,12 +505,12 @@ __device__ __forceinline__ void acc_unit(const InvArgs &a, int d, int b, int c, 
             atomicAdd(&hcnt[cur], ccnt);
         }
     }
-    __syncthreads();
+    acc_barrier<ONE_OUT>();
     if (f16 && tid == 0) *pw_lds = 0.f;
     if constexpr (Q == 1)
         if (next) acc_slot_table<R>(a, bn, slot_next);
     if (fold) fold_flush(a, d, hsum, hcnt, tid, WG);
-    __syncthreads();
+    acc_barrier<ONE_OUT>();
 }
 
 } // namespace msosm

@@ -172,3 +172,23 @@ def test_sub_accum_odd_batches_match_oracle_and_launch_forms(log2L, batch):
         err = rel_l2(got[d], ref[d])
         print(f"  DM {design.dms[d]}: rel-L2 {err:.3e}", flush=True)
         assert err < TOL
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+def test_sub_accum_one_output_launch_matches_general_launch(log2L):
+    # a plan with one output and no fold runs the sub-unit launch that loads each next spectrum into
+    # registers ahead of its stores; with a fold attached the general launch runs: the series agree bit
+    # for bit, for complex64 and for uint16 output
+    design = _design(log2L)
+    batch, calls = 2, 3
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, float(design.dms[2]), 0.0011, seed=140 + log2L)
+    raw = np.ascontiguousarray(raw[:n])
+    for out in (ms.OUT_C64, ms.OUT_U16):
+        print(f"case sub-unit accumulate L=2^{log2L} out={out}: one-output launch vs general launch", flush=True)
+        one = _run(design, raw, ms.IN_C64, batch, out)
+        fold = ms.Fold(0.0011, BW, per, 64, num_series=len(design.dms))
+        gen = _run(design, raw, ms.IN_C64, batch, out, fold=fold)
+        fold.close()
+        assert np.array_equal(one, gen)
