@@ -414,6 +414,9 @@ extern "C" int msosm_plan_create(const msosm_plan_desc *d, msosm_plan **out)
         return set_error(MSOSM_ERR_UNSUPPORTED, "plan_create: FFT length 2M=" + std::to_string(2LL * d->M) +
                                                      " outside the supported range [2^12, 2^22]");
     if (d->num_dms <= 0 || d->batch <= 0) return set_error(MSOSM_ERR_ARG, "plan_create: num_dms and batch must be > 0");
+    // unit (and sub-unit) positions of the dedispersion launches are 32-bit
+    if ((long long)d->num_dms * d->batch * 4 >= (1ll << 31))
+        return set_error(MSOSM_ERR_ARG, "plan_create: num_dms * batch must be below 2^29");
     if (d->delaycount <= 0 || d->delaycount > 65535)
         return set_error(MSOSM_ERR_ARG, "plan_create: delaycount must be in [1, 65535]");
     if (d->in_format < MSOSM_IN_U16 || d->in_format > MSOSM_IN_C64)

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(WG) inv_kernel(InvArgs a)
     fold_lds_init(hsum, hcnt, pw_lds, a.fold_sum ? a.nbins : 0);
     const UnitOrder o(a, gridDim.x, blockIdx.x);
 #pragma unroll 1
-    for (long long s = o.first; s < o.end; s += o.step) {
+    for (int s = o.first; s < o.end; s += o.step) {
         int d, b;
         if (o.at(s, a, d, b)) // uniform over the workgroup
             inv_unit<N1, N2, RING32, VAR, PREFETCH, SUBT, PREFETCH_DELAYS>(a, d, b, lds_all, tw, scr, hsum, hcnt,

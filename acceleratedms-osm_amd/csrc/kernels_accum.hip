@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
     extern __shared__ float2 lds_all[];
     uint32_t *slots = reinterpret_cast<uint32_t *>(lds_all + ACC_AREA + A::TW_WORDS); // two tables of 256
     const UnitOrder o(a, gridDim.x, blockIdx.x);
-    long long s = o.first;
+    int s = o.first;
     int d = 0, b = 0;
     bool have = o.next(s, a, d, b); // uniform over the workgroup
     if (have) acc_slot_table<R>(a, b, slots); // made visible by the twiddle load's barrier

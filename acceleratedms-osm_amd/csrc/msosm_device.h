@@ -677,10 +677,13 @@ __device__ __forceinline__ void inv_unit(const InvArgs &a, int d, int b, float2 
 //          DM's chirp and delays and each block's ring slots are shared in that XCD's L2;
 //   XCD DM-/block-major: workgroups w, w+8, ... share an XCD and get consecutive virtual ids;
 //   linear: workgroup w runs units w, w+G, ...
+// Positions are 32-bit (plans keep num_dms x batch x 4 below 2^31) and wave-uniform, kept in scalar
+// registers: wide or vector loop state is what the register allocator spills, and a spill reload at the
+// loop head waits for every outstanding store of the wave.
 struct UnitOrder {
     int dg, bg, j, tb;
     bool tiles, block_major;
-    long long first, end, step;
+    int first, end, step;
     __device__ UnitOrder(const InvArgs &a, int G, int w)
     {
         const int order = a.order & MSOSM_ORDER_MASK;
@@ -696,27 +699,29 @@ struct UnitOrder {
         j = w / 8;
         tb = (a.B + bg - 1) / bg;
         const int td = tiles ? (a.D + dg - 1) / dg : 0;
-        first = tiles ? w % 8 : xcd ? (w % 8) * (G / 8) + w / 8 : w;
-        end = tiles ? (long long)td * tb : (long long)a.D * a.B;
+        first = __builtin_amdgcn_readfirstlane(tiles ? w % 8 : xcd ? (w % 8) * (G / 8) + w / 8 : w);
+        end = __builtin_amdgcn_readfirstlane(tiles ? td * tb : a.D * a.B);
         step = tiles ? 8 : G;
     }
     // the unit at position s; false when the position names none
-    __device__ bool at(long long s, const InvArgs &a, int &d, int &b) const
+    __device__ bool at(int s, const InvArgs &a, int &d, int &b) const
     {
         if (tiles) {
-            d = (int)(s / tb) * dg + j / bg;
-            b = (int)(s % tb) * bg + j % bg;
+            d = (s / tb) * dg + j / bg;
+            b = (s % tb) * bg + j % bg;
         } else if (block_major) {
-            d = (int)(s % a.D);
-            b = (int)(s / a.D);
+            d = s % a.D;
+            b = s / a.D;
         } else {
-            d = (int)(s / a.B);
-            b = (int)(s % a.B);
+            d = s / a.B;
+            b = s % a.B;
         }
+        d = __builtin_amdgcn_readfirstlane(d);
+        b = __builtin_amdgcn_readfirstlane(b);
         return d < a.D && b < a.B;
     }
     // the next unit from position s on (s moves past it); false at the end of the sequence
-    __device__ bool next(long long &s, const InvArgs &a, int &d, int &b) const
+    __device__ bool next(int &s, const InvArgs &a, int &d, int &b) const
     {
         for (; s < end; s += step)
             if (at(s, a, d, b)) {
@@ -747,7 +752,7 @@ __device__ __forceinline__ void inv_rs(const InvArgs &a, float2 *lds_all, const 
     constexpr int BPS = 2 * (PassCount<N>::value - 1) + 2;
     const UnitOrder o(a, gridDim.x, blockIdx.x);
     const int role = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 9);
-    long long s = o.first;
+    int s = o.first;
     int d = 0, b = 0;
     // the unit loops are unrolled by two, one body per slot layout, so that every slot address is
     // a compile-time function of the thread's indices (a runtime layout costs 8 % more VALU). The
