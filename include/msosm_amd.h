@@ -98,7 +98,7 @@ typedef struct msosm_plan msosm_plan;
 typedef struct {
     int M;          /* half FFT length (fftpoint = 2M, a power of two in [2^12, 2^22]) */
     int num_dms;    /* DM trials D */
-    int batch;      /* blocks of M samples per push (B) */
+    int batch;      /* blocks of M samples per push (B); num_dms * batch below 2^29 */
     int delaycount; /* max over DMs of delaycount; spectra ring = roundup(delaycount + B - 1, B) slots */
     int in_format;  /* MSOSM_IN_* */
     int out_flags;  /* MSOSM_OUT_* */

@@ -97,6 +97,10 @@ def test_c_abi_argument_errors():
     with pytest.raises(ms.MsosmError) as e:
         ms.Plan(M=4096, num_dms=1, batch=1, delaycount=1, in_format=7)
     assert e.value.code == ms.ERR_ARG
+    # unit positions of the launches are 32-bit: num_dms x batch stays below 2^29
+    with pytest.raises(ms.MsosmError) as e:
+        ms.Plan(M=4096, num_dms=1 << 15, batch=1 << 14, delaycount=1)
+    assert e.value.code == ms.ERR_ARG and "2^29" in str(e.value)
 
 
 def test_uneven_order_rule():
