@@ -42,10 +42,11 @@ def order_code(name: str) -> int:
 # Uneven grid (--workload uneven): complex64 baseband, 2^26 samples at 32 MS/s, band 784-816 MHz,
 # 1024 DM trials over 0-1000 pc/cm^3, each at its own FFT length (the reference's minimum order for
 # that DM, 2^14..2^18 here), complex64 series out; one launch over all lengths vs one per DM. A push is
-# 16 blocks of the longest length's M = 2^17 (2^21 samples: the 2^17 class then runs 32-block pushes, the
-# reference speed harness's batch, test_msosm_speed.cpp:119-120).
+# 32 blocks of the longest length's M = 2^17 (2^22 samples), the reference speed harness's batch
+# (test_msosm_speed.cpp:119-120), as is the one-launch-per-DM form it is compared with; the sub-unit
+# spectra of one push take 60 GB of device memory.
 UNEVEN = dict(workload="msosm_uneven_c64", samples=1 << 26, fs_hz=32e6, fc_hz=800e6, bw_hz=32e6, num_dms=1024,
-              dm_lo=0.0, dm_hi=1000.0, push_blocks=16, output="complex64 series", input="complex64, normal")
+              dm_lo=0.0, dm_hi=1000.0, push_blocks=32, output="complex64 series", input="complex64, normal")
 
 # DM refinement (--workload refine): complex64 baseband, 2^24 samples at 16 MS/s, band 1392-1408 MHz,
 # a dispersed pulsar (DM 123.456, P = 33 ms, Gaussian pulse of 3 % FWHM, amplitude 0.5, AWGN sigma 1),
@@ -116,7 +117,7 @@ def main():
                     help="fixed unit order instead of the autotuner (profiling runs)")
     ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--push-blocks", type=int, default=0,
-                    help="uneven workload: blocks of the longest length per push (default 16)")
+                    help="uneven workload: blocks of the longest length per push (default 32)")
     ap.add_argument("--graph", action="store_true", help="uneven workload: run each one-launch push as a hipGraph")
     ap.add_argument("--plain-pushes", type=int, default=4,
                     help="uneven workload: pushes timed in the one-launch-per-DM form (at most a quarter record)")
