@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
                 sb[c * 4 * 64 * 4] = v.x; // rows_lds[buf][4 c + r][l], half e
                 sb[c * 4 * 64 * 4 + 1] = v.y;
             }
-            __syncthreads(); // the rows of this block are in LDS; the other buffer's readers are done
+            lds_barrier(); // the rows of this block are in LDS; the other buffer's readers are done
             float2 *Xb = X + (size_t)b * L;
 #pragma unroll
             for (int k = 0; k < ROWS / 8; ++k) {
