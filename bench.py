@@ -434,6 +434,10 @@ def bench_uneven(args, nested: bool = False):
         log(f"uneven per-DM launches at {pb} blocks: {per_dm[pb]['rate']:.3e} DM-trial*samples/s")
         m2.close()
     best_pb = max(per_dm, key=lambda k: per_dm[k]["rate"])
+    # the reference's per-DM structure with the vendor FFT library, at the harness's 32 blocks per call
+    lib_fft = per_dm_library_fft(design, data, 32)
+    log(f"uneven per-DM form with library FFTs at 32 blocks: {lib_fft['value']:.3e} DM-trial*samples/s "
+        f"({lib_fft['trials_timed']} trials timed), rel-L2 {max(lib_fft['rel_l2_per_length'].values()):.2e}")
     per_sample = alg_bytes_per_dm_sample(8, 8, D, int(2 * design.M.max()), N)
     alg = D * per * per_sample
     # dedispersion time per push: one launch, or one launch per class
@@ -454,7 +458,8 @@ def bench_uneven(args, nested: bool = False):
              "best_batch": {"push_blocks": best_pb, "value": per_dm[best_pb]["rate"],
                             "pushes": per_dm[best_pb]["pushes"],
                             "speedup_of_one_launch": one["rate"] / per_dm[best_pb]["rate"]},
-             "tried": {str(k): v["rate"] for k, v in per_dm.items()}}
+             "tried": {str(k): v["rate"] for k, v in per_dm.items()},
+             "library_fft_per_dm": dict(lib_fft, speedup_of_one_launch=one["rate"] / lib_fft["value"])}
     out = {
         "metric": "DM-trial*samples/s", "value": one["rate"], "unit": "DM-trial*samples/s",
         "ms_per_step": one["ms_per_push"] * pushes, "steps": args.steps,
@@ -469,6 +474,74 @@ def bench_uneven(args, nested: bool = False):
     out.update({"n_gpus": 1, "warmup": 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic", "cpu_baseline": None})
     print(json.dumps(out), flush=True)
+
+
+def per_dm_library_fft(design, data, push_blocks: int, pushes: int = 2, stride: int = 8) -> dict:
+    """The reference's GPU form on this GPU, one DM trial at a time (msosm_gpu_batch.cpp:193-201): per trial
+    and push, every bin's spectrum gathered from the ring of forward transforms by the bin's delay in
+    blocks, times the chirp, the push's blocks inverse-transformed as one batched FFT of the vendor library
+    (torch.fft on hipFFT/rocFFT), and each block's first half discarded into the trial's series. The
+    forward transforms are computed once per push and length class and shared by its trials (the
+    reference's instances each compute their own). Timed over `pushes` pushes after a warm-up push, on
+    every `stride`-th trial of each length class; the first push (zero history) of one trial per class is
+    compared with the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc
+
+    dev = data.device
+    Mmax = int(design.M.max())
+    per = push_blocks * Mmax
+    n_push = pushes + 1
+    x = data[: n_push * per]
+    classes = []
+    for M in np.unique(design.M):
+        M = int(M)
+        L, B = 2 * M, per // M
+        sel = np.nonzero(design.M == M)[0][::stride]
+        dmax = int(max(int(design.delays[i].max()) for i in sel))
+        # ring of forward transforms: row dmax + j is block j's segment [x_(j-1), x_j] (zero before the
+        # record); rows below dmax stay zero (blocks before the record)
+        ring = torch.zeros(dmax + n_push * B, L, dtype=torch.complex64, device=dev)
+        xpad = torch.cat([torch.zeros(M, dtype=torch.complex64, device=dev), x])
+        segs = xpad.unfold(0, L, M)
+        rel = [torch.from_numpy(np.arange(B)[:, None] + dmax - design.delays[i][None, :].astype(np.int64)).to(dev)
+               for i in sel]
+        H = [torch.from_numpy(np.ascontiguousarray(design.H[i]).astype(np.complex64)).to(dev) for i in sel]
+        out = torch.empty(len(sel), per, dtype=torch.complex64, device=dev)
+        classes.append(dict(M=M, L=L, B=B, sel=sel, dmax=dmax, ring=ring, segs=segs, rel=rel, H=H, out=out))
+
+    def push(p, keep=None):
+        for c in classes:
+            M, L, B = c["M"], c["L"], c["B"]
+            c["ring"][c["dmax"] + p * B: c["dmax"] + (p + 1) * B] = torch.fft.fft(c["segs"][p * B:(p + 1) * B], dim=1)
+            for j in range(len(c["sel"])):
+                Y = torch.gather(c["ring"], 0, c["rel"][j] + p * B) * c["H"][j]
+                y = torch.fft.ifft(Y, dim=1)
+                c["out"][j].view(B, M).copy_(y[:, M:])
+            if keep is not None:
+                keep[M] = c["out"][-1].cpu().numpy().copy()
+
+    first = {}
+    push(0, first)  # warm-up, and the checked push
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(1, n_push):
+        push(p)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    xs = x[:per].cpu().numpy().astype(np.complex128)
+    errs = {}
+    for c in classes:
+        i = int(c["sel"][-1])
+        ref = orc.msosm(xs, c["M"], design.delays[i][None], design.H[i][None].astype(np.complex128))[0]
+        errs[str(c["L"])] = rel_err_series(first[c["M"]], ref)[0]
+    trials = sum(len(c["sel"]) for c in classes)
+    del classes
+    torch.cuda.empty_cache()
+    return {"value": trials * per * pushes / dt, "trials_timed": trials, "push_blocks": push_blocks,
+            "pushes": pushes, "rel_l2_per_length": errs}
 
 
 def rel_err_uneven(ms, design, multi, data) -> dict:
