@@ -137,7 +137,8 @@ struct AccR {
 // the series at about the same time and the line leaves the L2 whole. The outputs do not depend on it.
 // OUTK = ACC_OUT_C64 or ACC_OUT_U16 (sub-units with that one output and no fold, ONE_OUT): the item's first
 // spectrum arrives in registers (xv, loaded by the previous item or the kernel), and the next item's
-// (xn, when there is one) is loaded into xv just before this item's output stores, so that the next
+// (xn; after the last item this item's own again) is loaded into xv just before this item's output
+// stores, so that the next
 // item's first transform waits for those loads only and not for the stores (a wave's loads and stores
 // share its memory counter, which the compiler's waits count precisely for register loads); the
 // unit's barriers order LDS only, so the stores stay in flight through that transform.
