@@ -145,3 +145,41 @@ def test_graph_push_replays_match_eager():
         assert np.array_equal(res[0][0][c], res[1][0][c]), f"push {c}"
     assert np.array_equal(res[0][2], res[1][2])
     assert np.allclose(res[0][1], res[1][1], rtol=1e-12, atol=0)
+
+
+def test_tuned_grid_at_the_bench_shape_matches_oracle():
+    # the uneven bench's shape: band 784-816 MHz, pushes of 32 blocks of the longest length (2^22
+    # samples), complex64 output, every length class's schedule tuned by msosm_multi_autotune (which
+    # leaves the grid at the stream start); one DM of each class 2^14..2^18 against the FP64 oracle
+    # over two pushes, in the one-launch form and in the one-launch-per-class form
+    import torch
+
+    bw, f0, period = 32e6, 784e6, 0.033
+    design = ms.UnevenDesign(bw, f0, [4.0, 20.0, 80.0, 300.0, 900.0])
+    assert sorted((2 * design.M).tolist()) == [1 << k for k in range(14, 19)]
+    per, calls = 1 << 22, 2
+    raw, x = make_input(ms.IN_C64, per * calls, bw, f0, 300.0, period, seed=35)
+    dev = torch.from_numpy(raw.view(np.float32)).cuda()
+    torch.cuda.synchronize()
+    refs = []
+    for i in range(len(design.dms)):
+        M, dm = int(design.M[i]), float(design.dms[i])
+        d, _ = orc.segmentation(bw, f0, dm, M)
+        refs.append(orc.msosm(x, M, d[None], orc.transfer_function(bw, f0, dm, M)[None])[0])
+    for mode in (ms.LAUNCH_PERSISTENT, ms.LAUNCH_PER_CLASS):
+        multi = ms.Multi(design, push_blocks=32, out_flags=ms.OUT_C64, device=0)
+        assert multi.push_samples == per
+        best, tms = multi.autotune(dev.data_ptr(), calls)
+        multi.set_launch_mode(mode)
+        print(f"case tuned uneven grid at 32 blocks per push, launch mode {mode} (tuner's pick {best}, "
+              f"{tms:.2f} ms per push)", flush=True)
+        outs = [[] for _ in range(multi.D)]
+        for c in range(calls):
+            multi.push(dev.data_ptr() + c * per * 8)
+            for i in range(multi.D):
+                outs[i].append(multi.output(i))
+        multi.close()
+        for i, ref in enumerate(refs):
+            err = rel_l2(np.concatenate(outs[i]), ref)
+            print(f"  DM {design.dms[i]} (L = {2 * int(design.M[i])}): rel-L2 {err:.2e}", flush=True)
+            assert err < TOL
