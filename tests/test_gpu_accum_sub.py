@@ -192,3 +192,38 @@ def test_sub_accum_one_output_launch_matches_general_launch(log2L):
         gen = _run(design, raw, ms.IN_C64, batch, out, fold=fold)
         fold.close()
         assert np.array_equal(one, gen)
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+def test_sub_accum_series_across_ring_wrap(log2L):
+    # enough pushes that the input ring (roundup(delaycount + batch - 1, batch) blocks) wraps: the gathering
+    # pass reads the history through the wrapped slots; the one-output launch's series of three trials at
+    # the low end of the length class against the oracle, before and after the first wrap
+    dms = {17: [130.0, 135.0, 140.0], 18: [506.0, 510.0, 515.0]}[log2L]
+    design = ms.Design(BW, F0, np.asarray(dms, dtype=np.float32))
+    assert 2 * design.M == 1 << log2L
+    batch = 4
+    per = batch * design.M
+    plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64)
+    plan.set_schedule(0, ACC)
+    fbs = plan.info().FBS
+    assert fbs == -(-(design.delaycount + batch - 1) // batch) * batch
+    calls = -(-3 * fbs // (2 * batch))  # 1.5 ring depths of blocks
+    n = per * calls
+    raw, x = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, dms[1], 0.0011, seed=150 + log2L)
+    raw, x = np.ascontiguousarray(raw[:n]), x[:n]
+    print(f"case sub-unit accumulate across the ring wrap: L=2^{log2L} dc={design.delaycount} ring={fbs} blocks "
+          f"pushes={calls}", flush=True)
+    outs = []
+    for c in range(calls):
+        plan.push(raw[c * per:(c + 1) * per])
+        outs.append(plan.output_c64())
+    plan.close()
+    got = np.concatenate(outs, axis=1)
+    delays, H = oracle_design(BW, F0, design.dms, design.M)
+    ref = orc.msosm(x, design.M, delays, H)
+    wrap = fbs * design.M  # first sample whose block reuses a ring slot
+    for d in range(len(dms)):
+        e0, e1 = rel_l2(got[d, :wrap], ref[d, :wrap]), rel_l2(got[d, wrap:], ref[d, wrap:])
+        print(f"  DM {dms[d]}: rel-L2 before the wrap {e0:.2e}, after {e1:.2e}", flush=True)
+        assert e0 < TOL and e1 < TOL
