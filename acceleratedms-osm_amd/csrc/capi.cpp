@@ -771,7 +771,7 @@ int msosm::plan_push_prepare(msosm_plan *p, const void *src, InvArgs *out, bool 
     plan_push_args(p, src, &a, allow_sub);
     EventPair ev{};
     if (p->profile && record_begin(p, ev) != MSOSM_OK) return MSOSM_ERR_HIP;
-    HIPCHK(launch_fwd(a.log2L, a.fwd, a.fwd_grid, p->stream));
+    HIPCHK(launch_fwd(a.log2L, a.fwd, a.fwd_grid, p->G, p->stream));
     if (p->profile) {
         HIPCHK(hipEventRecord(ev.b, p->stream));
         p->pending[0].push_back(ev);

@@ -155,7 +155,9 @@ int tiles_lds_bytes(int log2L); // tiles and LDS twiddle tables of the forward a
 constexpr int INV_EXTRA_LDS = 16;
 int workgroup_threads();
 hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes);
-hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st);
+// grid: fwd_kernel's workgroups (one scratch slot each); cus: workgroups the GPU holds at once, the
+// bound of the split form's B P workgroups (used when grid = B)
+hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, int cus, hipStream_t st);
 hipError_t launch_inv(int log2L, const InvArgs &a, int grid, hipStream_t st);
 hipError_t launch_hist(float2 *hist, const void *in, int fmt, long long first, int M, hipStream_t st);
 hipError_t launch_bins(uint16_t *bins, long long n, unsigned long nbins, double t_start, double dt, double period,

@@ -10,46 +10,89 @@ namespace msosm {
 // ---------------------------------------------------------------------------------------------
 // Forward: unpack + window + DFT_L into ring slots. One unit = one block of M new samples.
 // ---------------------------------------------------------------------------------------------
+// Pass A of block b's transform (tiles [t_begin, t_end)): the block's L input samples (the M before it
+// from the history at the stream start) into the block's scratch slot.
+template <int N1, int N2>
+__device__ __forceinline__ void fwd_block_a(const FwdArgs &a, int b, float2 *scr, float2 *lds_all, const Tw &tw,
+                                            int t_begin, int t_end)
+{
+    constexpr int M = Shape<N1, N2>::L / 2;
+    const long long base = (long long)(b - 1) * M;
+    auto load = [&](int k) -> float2 {
+        const long long s = base + k;
+        if (s < 0) return a.hist[s + M];
+        return decode_in(a.in, a.in_fmt, s);
+    };
+    pass_a<N1, N2, -1>(load, scr, lds_all, tw, t_begin, t_end);
+}
+
+// Pass B of block b's transform (tiles [t_begin, t_end)): from the scratch slot into the block's ring
+// slot (and the permuted ring, when the plan keeps one).
+template <int N1, int N2>
+__device__ __forceinline__ void fwd_block_b(const FwdArgs &a, int b, const float2 *scr, float2 *lds_all,
+                                            const Tw &tw, int t_begin, int t_end)
+{
+    using S = Shape<N1, N2>;
+    constexpr int L = S::L;
+    const long long slot = (a.g0 + b) % a.FBS;
+    float2 *dst = a.ring + (size_t)slot * L;
+    float2 *dstp = a.ring_p ? a.ring_p + (size_t)slot * L : nullptr;
+    auto prep = [](int, bool) { return 0; };
+    auto emit = [&](const float2 *lds, int tile, bool active, int) {
+        if (!active) return;
+        constexpr int NC = S::NCB;
+        const int lt = threadIdx.x & 255;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int f = lt + 256 * i;
+            const int n1 = f / NC, row = f % NC;
+            const uint32_t k = (uint32_t)(tile * NC + row + N2 * n1);
+            const float2 y = lds[lds_pad(lt) + 272 * i]; // pad(lt + 256 i)
+            st32(dst, k, y);
+            if (dstp) st32(dstp, perm_pos(k, a.perm_rl, a.perm_sl), y);
+        }
+    };
+    pass_b<N1, N2, -1>(scr, prep, emit, lds_all, tw, 0, t_begin, t_end);
+}
+
+// One workgroup per block (blocks b, b + grid, ...), both passes through the workgroup's scratch slot.
 template <int N1, int N2>
 __global__ void __launch_bounds__(WG) fwd_kernel(FwdArgs a)
 {
     using S = Shape<N1, N2>;
-    constexpr int L = S::L, M = L / 2;
     extern __shared__ float2 lds_all[];
     Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
     load_twiddles<N1, N2>(lds_all, tw);
-    float2 *scr = a.scratch + (size_t)blockIdx.x * L;
+    float2 *scr = a.scratch + (size_t)blockIdx.x * S::L;
 #pragma unroll 1
     for (int b = blockIdx.x; b < a.B; b += gridDim.x) {
-        const long long base = (long long)(b - 1) * M;
-        auto load = [&](int k) -> float2 {
-            const long long s = base + k;
-            if (s < 0) return a.hist[s + M];
-            return decode_in(a.in, a.in_fmt, s);
-        };
-        pass_a<N1, N2, -1>(load, scr, lds_all, tw);
+        fwd_block_a<N1, N2>(a, b, scr, lds_all, tw, 0, S::NTA);
         __syncthreads();
-        const long long slot = (a.g0 + b) % a.FBS;
-        float2 *dst = a.ring + (size_t)slot * L;
-        float2 *dstp = a.ring_p ? a.ring_p + (size_t)slot * L : nullptr;
-        auto prep = [](int, bool) { return 0; };
-        auto emit = [&](const float2 *lds, int tile, bool active, int) {
-            if (!active) return;
-            constexpr int NC = S::NCB;
-            const int lt = threadIdx.x & 255;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int f = lt + 256 * i;
-                const int n1 = f / NC, row = f % NC;
-                const uint32_t k = (uint32_t)(tile * NC + row + N2 * n1);
-                const float2 y = lds[lds_pad(lt) + 272 * i]; // pad(lt + 256 i)
-                st32(dst, k, y);
-                if (dstp) st32(dstp, perm_pos(k, a.perm_rl, a.perm_sl), y);
-            }
-        };
-        pass_b<N1, N2, -1>(scr, prep, emit, lds_all, tw);
+        fwd_block_b<N1, N2>(a, b, scr, lds_all, tw, 0, S::NTB);
         __syncthreads();
     }
+}
+
+// The same transform with each block's passes split over P workgroups (a push of fewer blocks than
+// the GPU has CUs): workgroup (b, p) = (blockIdx.x / P, blockIdx.x mod P) runs its P-th share of the
+// tiles of pass PASS through block b's own scratch slot (the plan has one per block). Two launches,
+// so that every pass-A tile of a block is in its slot before any pass-B tile reads it; the tiles and
+// their arithmetic are fwd_kernel's, so the spectra are the same bits.
+template <int N1, int N2, int PASS>
+__global__ void __launch_bounds__(WG) fwd_split_kernel(FwdArgs a, int P)
+{
+    using S = Shape<N1, N2>;
+    extern __shared__ float2 lds_all[];
+    Tw tw{a.tw_n1, a.tw_n2, a.tw_lo, a.tw_hi, a.tw_s};
+    load_twiddles<N1, N2>(lds_all, tw);
+    const int b = (int)blockIdx.x / P, p = (int)blockIdx.x % P;
+    float2 *scr = a.scratch + (size_t)b * S::L;
+    constexpr int NT = PASS == 0 ? S::NTA : S::NTB;
+    const int share = NT / P;
+    if constexpr (PASS == 0)
+        fwd_block_a<N1, N2>(a, b, scr, lds_all, tw, p * share, (p + 1) * share);
+    else
+        fwd_block_b<N1, N2>(a, b, scr, lds_all, tw, p * share, (p + 1) * share);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -216,11 +259,35 @@ __global__ void fill_kernel(uint32_t *p, long long nwords, uint32_t value)
 // ---------------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------------
+// Workgroups per block of the split forward transform: the largest power of two that divides both
+// passes' tile steps and keeps B P within the workgroups the GPU holds at once; 1 = no split.
 template <int N1, int N2>
-static hipError_t launch_fwd_t(const FwdArgs &a, int grid, hipStream_t st)
+static int fwd_split_ways(int B, int cus)
 {
     using S = Shape<N1, N2>;
-    hipLaunchKernelGGL((fwd_kernel<N1, N2>), dim3(grid), dim3(WG), S::LDS_BYTES, st, a);
+    const int steps = (S::NTA < S::NTB ? S::NTA : S::NTB) / SUBT;
+    if (S::NTA % SUBT || S::NTB % SUBT) return 1;
+    int P = 1;
+    while (2 * P <= steps && (long long)B * 2 * P <= cus) P *= 2;
+    return P;
+}
+
+// grid workgroups of fwd_kernel, or, when every block has a scratch slot of its own (grid = B) and the
+// blocks alone leave CUs idle, the two split launches over B P workgroups
+template <int N1, int N2>
+static hipError_t launch_fwd_t(const FwdArgs &a, int grid, int cus, hipStream_t st)
+{
+    using S = Shape<N1, N2>;
+    const int P = grid == a.B ? fwd_split_ways<N1, N2>(a.B, cus) : 1;
+    if (P == 1) {
+        hipLaunchKernelGGL((fwd_kernel<N1, N2>), dim3(grid), dim3(WG), S::LDS_BYTES, st, a);
+        return hipGetLastError();
+    }
+    const dim3 g((unsigned)(a.B * P));
+    hipLaunchKernelGGL((fwd_split_kernel<N1, N2, 0>), g, dim3(WG), S::LDS_BYTES, st, a, P);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fwd_split_kernel<N1, N2, 1>), g, dim3(WG), S::LDS_BYTES, st, a, P);
     return hipGetLastError();
 }
 
@@ -297,11 +364,11 @@ int tiles_lds_bytes(int log2L)
     return -1;
 }
 
-hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, hipStream_t st)
+hipError_t launch_fwd(int log2L, const FwdArgs &a, int grid, int cus, hipStream_t st)
 {
     switch (log2L) {
 #define X(lg, a_, b_)                                                                                                  \
-    case lg: return launch_fwd_t<a_, b_>(a, grid, st);
+    case lg: return launch_fwd_t<a_, b_>(a, grid, cus, st);
         MSOSM_SHAPES(X)
 #undef X
     }
@@ -443,9 +510,14 @@ hipError_t configure_lds(int log2L, int fwd_bytes, int inv_bytes)
     switch (log2L) {
 #define X(lg, a_, b_)                                                                                                  \
     case lg: {                                                                                                         \
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&fwd_kernel<a_, b_>),                        \
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, fwd_bytes);                     \
-        if (e != hipSuccess) return e;                                                                                 \
+        const void *fwd[] = {reinterpret_cast<const void *>(&fwd_kernel<a_, b_>),                                      \
+                             reinterpret_cast<const void *>(&fwd_split_kernel<a_, b_, 0>),                             \
+                             reinterpret_cast<const void *>(&fwd_split_kernel<a_, b_, 1>)};                            \
+        hipError_t e = hipSuccess;                                                                                     \
+        for (const void *f : fwd) {                                                                                    \
+            e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fwd_bytes);                         \
+            if (e != hipSuccess) return e;                                                                             \
+        }                                                                                                              \
         const void *inv[] = {                                                                                          \
             reinterpret_cast<const void *>(&inv_kernel<a_, b_, true, VAR_PLAIN>),                                      \
             reinterpret_cast<const void *>(&inv_kernel<a_, b_, true, VAR_F16>),                                        \

@@ -183,8 +183,10 @@ __device__ __forceinline__ void pass_a_tile(float2 (&v)[16], float2 *__restrict_
 
 // With PF, the next tile's inputs are loaded while the current tile is transformed (32 more VGPRs):
 // the loads' latency then overlaps the sub-FFT instead of stalling every wave at the tile start.
+// Tiles [t_begin, t_end) (multiples of NSUB; by default all of them).
 template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, class Load>
-__device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ scr, float2 *lds_all, const Tw &tw)
+__device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ scr, float2 *lds_all, const Tw &tw,
+                                       int t_begin = 0, int t_end = Shape<N1, N2, NSUB>::NTA)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N2, NC = S::NCA, TPC = NS / 16;
@@ -205,14 +207,14 @@ __device__ __forceinline__ void pass_a(const Load &load, float2 *__restrict__ sc
         }
     };
     float2 nx[16];
-    if constexpr (PF) fetch(0, nx);
+    if constexpr (PF) fetch(t_begin, nx);
 #pragma unroll 1
-    for (int t0 = 0; t0 < S::NTA; t0 += NSUB) {
+    for (int t0 = t_begin; t0 < t_end; t0 += NSUB) {
         float2 v[16];
         if constexpr (PF) {
 #pragma unroll
             for (int t = 0; t < 16; ++t) v[t] = nx[t];
-            if (t0 + NSUB < S::NTA) fetch(t0 + NSUB, nx);
+            if (t0 + NSUB < t_end) fetch(t0 + NSUB, nx);
         } else {
             fetch(t0, v);
         }
@@ -363,9 +365,11 @@ __device__ __forceinline__ void pass_a_gather_pf(const float2 *__restrict__ ring
 // (global loads the epilogue needs: they retire before the prefetch, so waiting for them does not
 // wait for it); its result is handed to emit(lds, tile, active, prepared).
 // SUB0: the first of this pass's 256-thread groups (the role-split kernel runs pass B on groups 2, 3).
+// Tiles [t_begin, t_end) (multiples of NSUB; by default all of them).
 template <int N1, int N2, int DIR, bool PF = false, int NSUB = SUBT, int SUB0 = 0, class Prep, class Emit>
 __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Prep &prep, const Emit &emit,
-                                       float2 *lds_all, const Tw &tw, int layout = 0)
+                                       float2 *lds_all, const Tw &tw, int layout = 0, int t_begin = 0,
+                                       int t_end = Shape<N1, N2, NSUB>::NTB)
 {
     using S = Shape<N1, N2, NSUB>;
     constexpr int NS = N1, NC = S::NCB, TPC = NS / 16, NCA = S::NCA;
@@ -389,9 +393,9 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
         }
     };
     float2 nx[16];
-    if constexpr (PF) fetch(0, nx);
+    if constexpr (PF) fetch(t_begin, nx);
 #pragma unroll 1
-    for (int t0 = 0; t0 < S::NTB; t0 += NSUB) {
+    for (int t0 = t_begin; t0 < t_end; t0 += NSUB) {
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int sub = (tid >> 8) - SUB0, lt = tid & 255;
@@ -408,7 +412,7 @@ __device__ __forceinline__ void pass_b(const float2 *__restrict__ scr, const Pre
             for (int t = 0; t < 16; ++t) v[t] = nx[t];
             butterflies<NS, 16, 1, DIR>(v, 0, tw.n1);
             pin16(v);
-            if (t0 + NSUB < S::NTB) fetch(t0 + NSUB, nx);
+            if (t0 + NSUB < t_end) fetch(t0 + NSUB, nx);
             __builtin_amdgcn_sched_barrier(0);
         } else {
             fetch(t0, v);
