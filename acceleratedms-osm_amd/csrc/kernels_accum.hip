@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
 // r 2^14 + 1024 w + 128 i + 2 l + e, e = 0 for bin k', 1 for k' + 4096). The workgroup takes one w at a
 // time: its 512 threads gather and transform one bin each (threads 256 e + 128 h + m: bin
 // k' = 8192 i + 4 base(w) + 512 h + m + 4096 e with base(w) = 32 (w div 4) + 256 (w mod 4), i.e. r = m mod 4,
-// word l = m div 4 + 32 h), put the values of each block into LDS in the rows' order, and each wave
-// stores whole 1 KiB rows.
+// word l = m div 4 + 32 h), put the values of 4 / Q blocks at a time (16 rows) into LDS in the rows'
+// order, and each wave stores two whole 1 KiB rows.
 // The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
 // factors) and its powers c = 2, 3 from products of it.
 // Workgroups take the DMs fastest (neighbouring workgroups gather the same bins of one ring for different
@@ -73,10 +73,11 @@ template <int Q>
 __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 {
     static_assert(Q == 2 || Q == 4, "two or four sub-units");
-    constexpr int LS = 1 << 16, ROWS = 4 * Q; // rows (c, r) of one w
+    constexpr int LS = 1 << 16, ROWS = 4 * Q; // rows (c, r) of one w and one block
+    constexpr int NBB = 4 / Q;                // blocks per LDS buffer: 16 rows, two per wave
     constexpr long long L = (long long)Q * LS;
-    // two buffers of the rows (c, r) of one w: 64 words of 16 bytes each
-    __shared__ float4 rows_lds[2][ROWS][64];
+    // two buffers of the rows (j, c, r) of one w for blocks b0 + j: 64 words of 16 bytes each
+    __shared__ float4 rows_lds[2][16][64];
     const int d = (int)(blockIdx.x % a.D), chunk = (int)(blockIdx.x / a.D);
     const int tid = (int)threadIdx.x;
     // the sub-unit launch's arrival counters of this DM's units start at zero
@@ -108,11 +109,11 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
         for (int c = 1; c < Q; ++c) wc[c] = c == 1 ? w1 : cmul(wc[c - 1], w1);
         // row (c, rr) of this w, word lane: position c 2^16 + rr 2^14 + 1024 w + 128 chunk + 2 lane
         const size_t row0 = (size_t)1024 * w + 128 * chunk + 2 * lane;
-        // Q = 2: the ring values of block b + 1 are requested before block b's rows are stored (a load
-        // issued after the stores would wait for them: they share the wave's memory counter); after the
-        // last block the request repeats it (in bounds, unused). Measured: 3.78 -> 3.46 ms per launch at
-        // Q = 2 (387 trials x 32 blocks), but 4.03 -> 4.33 ms at Q = 4, which gathers each block when
-        // it needs it.
+        // Q = 2: the ring values of the next NBB blocks are requested before this step's rows are stored
+        // (a load issued after the stores would wait for them: they share the wave's memory counter);
+        // after the last step the request repeats it (in bounds, unused). Measured at one block per
+        // step: 3.78 -> 3.46 ms per launch at Q = 2 (387 trials x 32 blocks), but 4.03 -> 4.33 ms at
+        // Q = 4, which gathers each block when it needs it.
         constexpr bool ahead = Q == 2;
         auto gather = [&](int b, float2 *y) {
 #pragma unroll
@@ -122,35 +123,48 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
                 y[s] = a.ring[sl * L + kp + (long long)s * LS];
             }
         };
-        float2 yn[Q];
-        if constexpr (ahead) gather(0, yn);
+        // blocks b0 .. b0 + NBB - 1; indices past the last block repeat it (in bounds; never stored)
+        auto gather_step = [&](int b0, float2 (&yy)[NBB][Q]) {
+#pragma unroll
+            for (int j = 0; j < NBB; ++j) gather(b0 + j < a.B ? b0 + j : a.B - 1, yy[j]);
+        };
+        float2 yn[NBB][Q];
+        if constexpr (ahead) gather_step(0, yn);
 #pragma unroll 1
-        for (int b = 0; b < a.B; ++b) {
-            float2 y[Q];
+        for (int b0 = 0; b0 < a.B; b0 += NBB) {
+            float2 y[NBB][Q];
             if constexpr (ahead) {
 #pragma unroll
-                for (int s = 0; s < Q; ++s) y[s] = yn[s];
-                gather(b + 1 < a.B ? b + 1 : b, yn);
+                for (int j = 0; j < NBB; ++j)
+#pragma unroll
+                    for (int s = 0; s < Q; ++s) y[j][s] = yn[j][s];
+                gather_step(b0 + NBB < a.B ? b0 + NBB : b0, yn);
             } else {
-                gather(b, y);
+                gather_step(b0, y);
             }
+            float *sb = slot + buf * (16 * 64 * 4);
 #pragma unroll
-            for (int s = 0; s < Q; ++s) y[s] = cmul(y[s], hc[s]);
-            dft<+1, Q>(y);
-            float *sb = slot + buf * (ROWS * 64 * 4);
+            for (int j = 0; j < NBB; ++j) {
 #pragma unroll
-            for (int c = 0; c < Q; ++c) {
-                const float2 v = c == 0 ? y[0] : cmul(y[c], wc[c]);
-                sb[c * 4 * 64 * 4] = v.x; // rows_lds[buf][4 c + r][l], half e
-                sb[c * 4 * 64 * 4 + 1] = v.y;
+                for (int s = 0; s < Q; ++s) y[j][s] = cmul(y[j][s], hc[s]);
+                dft<+1, Q>(y[j]);
+#pragma unroll
+                for (int c = 0; c < Q; ++c) {
+                    const float2 v = c == 0 ? y[j][0] : cmul(y[j][c], wc[c]);
+                    sb[(j * ROWS + c * 4) * 64 * 4] = v.x; // rows_lds[buf][j ROWS + 4 c + r][l], half e
+                    sb[(j * ROWS + c * 4) * 64 * 4 + 1] = v.y;
+                }
             }
-            lds_barrier(); // the rows of this block are in LDS; the other buffer's readers are done
-            float2 *Xb = X + (size_t)b * L;
+            lds_barrier(); // the rows of this step are in LDS; the other buffer's readers are done
 #pragma unroll
-            for (int k = 0; k < ROWS / 8; ++k) {
-                const int row = wave + 8 * k; // 4 c + rr
-                st_stream(reinterpret_cast<float4 *>(Xb + (size_t)(row >> 2) * LS + (size_t)(row & 3) * (1 << 14) + row0),
-                          rows_lds[buf][row][lane]);
+            for (int k = 0; k < 2; ++k) {
+                const int row = wave + 8 * k; // j ROWS + 4 c + rr
+                const int j = row / ROWS, rw = row % ROWS;
+                if (b0 + j < a.B) {
+                    float2 *Xb = X + (size_t)(b0 + j) * L;
+                    st_stream(reinterpret_cast<float4 *>(Xb + (size_t)(rw >> 2) * LS + (size_t)(rw & 3) * (1 << 14) + row0),
+                              rows_lds[buf][row][lane]);
+                }
             }
             buf ^= 1;
         }
