@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(WG) acc_kernel(InvArgs a)
 // r 2^14 + 1024 w + 128 i + 2 l + e, e = 0 for bin k', 1 for k' + 4096). The workgroup takes one w at a
 // time: its 512 threads gather and transform one bin each (threads 256 e + 128 h + m: bin
 // k' = 8192 i + 4 base(w) + 512 h + m + 4096 e with base(w) = 32 (w div 4) + 256 (w mod 4), i.e. r = m mod 4,
-// word l = m div 4 + 32 h), put the values of 4 / Q blocks at a time (16 rows) into LDS in the rows'
-// order, and each wave stores two whole 1 KiB rows.
+// word l = m div 4 + 32 h), put the values of 8 / Q blocks at a time (32 rows) into LDS in the rows'
+// order, and each wave stores four whole 1 KiB rows per barrier.
 // The twiddle e^{+2 pi i k' / L} comes from the plan's split tables (one product of correctly rounded
 // factors) and its powers c = 2, 3 from products of it.
 // Workgroups take the DMs fastest (neighbouring workgroups gather the same bins of one ring for different
@@ -74,10 +74,11 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
 {
     static_assert(Q == 2 || Q == 4, "two or four sub-units");
     constexpr int LS = 1 << 16, ROWS = 4 * Q; // rows (c, r) of one w and one block
-    constexpr int NBB = 4 / Q;                // blocks per LDS buffer: 16 rows, two per wave
+    constexpr int NBB = 8 / Q;                // blocks per LDS buffer: 32 rows, four per wave
+    constexpr int NR = NBB * ROWS;
     constexpr long long L = (long long)Q * LS;
     // two buffers of the rows (j, c, r) of one w for blocks b0 + j: 64 words of 16 bytes each
-    __shared__ float4 rows_lds[2][16][64];
+    __shared__ float4 rows_lds[2][NR][64];
     const int d = (int)(blockIdx.x % a.D), chunk = (int)(blockIdx.x / a.D);
     const int tid = (int)threadIdx.x;
     // the sub-unit launch's arrival counters of this DM's units start at zero
@@ -142,7 +143,7 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
             } else {
                 gather_step(b0, y);
             }
-            float *sb = slot + buf * (16 * 64 * 4);
+            float *sb = slot + buf * (NR * 64 * 4);
 #pragma unroll
             for (int j = 0; j < NBB; ++j) {
 #pragma unroll
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(PRE_WG) acc_pre_kernel(InvArgs a)
             }
             lds_barrier(); // the rows of this step are in LDS; the other buffer's readers are done
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < NR / 8; ++k) {
                 const int row = wave + 8 * k; // j ROWS + 4 c + rr
                 const int j = row / ROWS, rw = row % ROWS;
                 if (b0 + j < a.B) {
