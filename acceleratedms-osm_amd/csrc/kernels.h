@@ -16,7 +16,7 @@ struct FwdArgs {
     int FBS;
     long long g0;          // stream block index of this push's first block
     int B;                 // blocks per push
-    float2 *scratch;       // [grid][L]
+    float2 *scratch;       // [grid][L]: a slot per workgroup (fwd_kernel) or per block (fwd_split_kernel)
     const float2 *tw_n1, *tw_n2, *tw_lo, *tw_hi;
     int tw_s;
     float2 *ring_p;        // [FBS][L] the same spectra in the accumulate layout (perm_pos), or null
