@@ -227,3 +227,31 @@ def test_sub_accum_series_across_ring_wrap(log2L):
         e0, e1 = rel_l2(got[d, :wrap], ref[d, :wrap]), rel_l2(got[d, wrap:], ref[d, wrap:])
         print(f"  DM {dms[d]}: rel-L2 before the wrap {e0:.2e}, after {e1:.2e}", flush=True)
         assert e0 < TOL and e1 < TOL
+
+
+@pytest.mark.parametrize("log2L", [17, 18])
+def test_sub_accum_schedule_falls_back_for_fp16_output(log2L):
+    # the sub-unit form has no FP16-pair epilogue (its per-block scale would span the Q sub-units): a plan
+    # with FP16 output under the accumulate schedule runs the two-pass form, bit for bit as a plan
+    # scheduled for it, FP16 pairs and block scales included
+    design = _design(log2L)
+    batch, calls = 2, 2
+    per = batch * design.M
+    n = per * calls
+    raw, _ = make_input(ms.IN_C64, 1 << (n - 1).bit_length(), BW, F0, float(design.dms[1]), 0.0011,
+                        seed=160 + log2L)
+    raw = np.ascontiguousarray(raw[:n])
+    res = []
+    for order in (ACC, ms.ORDER_XCD_DM_MAJOR):
+        print(f"case FP16 output L=2^{log2L} order={order}", flush=True)
+        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_C64, out_flags=ms.OUT_C64 | ms.OUT_F16)
+        plan.set_schedule(0, order)
+        outs = []
+        for c in range(calls):
+            plan.push(raw[c * per:(c + 1) * per])
+            outs.append((plan.output_c64(), *plan.output_f16()))
+        plan.close()
+        res.append(outs)
+    for a, b in zip(res[0], res[1]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
