@@ -332,8 +332,9 @@ def test_role_split_every_autotune_candidate(band):
                 np.testing.assert_allclose(sums, ref[1], rtol=1e-12)
 
 
+@pytest.mark.parametrize("fmt", [ms.IN_C64, ms.IN_I8, ms.IN_U16])
 @pytest.mark.parametrize("log2L", [15, 16, 18])
-def test_split_forward_transform_matches_one_workgroup_form(log2L):
+def test_split_forward_transform_matches_one_workgroup_form(log2L, fmt):
     # a push of fewer blocks than the GPU holds workgroups splits each block's forward transform over
     # several workgroups (two launches); with as many workgroups as blocks it runs one workgroup per
     # block. The tiles and their arithmetic are the same, so the series are the same bits.
@@ -341,12 +342,12 @@ def test_split_forward_transform_matches_one_workgroup_form(log2L):
     batch, calls = 3, 2
     per = batch * design.M
     n = per * calls
-    raw, _ = make_input(ms.IN_I8, 1 << (n - 1).bit_length(), BW, F0, 60.0, 0.0011, seed=60 + log2L)
+    raw, _ = make_input(fmt, 1 << (n - 1).bit_length(), BW, F0, 60.0, 0.0011, seed=60 + log2L)
     raw = np.ascontiguousarray(raw[:n])
     res = []
     for wg in (batch, 0):
-        print(f"case forward transform L=2^{log2L} batch={batch} workgroups={wg or 'all CUs'}", flush=True)
-        plan = ms.Plan.from_design(design, batch, in_format=ms.IN_I8, out_flags=ms.OUT_C64)
+        print(f"case forward transform L=2^{log2L} fmt={fmt} batch={batch} workgroups={wg or 'all CUs'}", flush=True)
+        plan = ms.Plan.from_design(design, batch, in_format=fmt, out_flags=ms.OUT_C64)
         plan.set_schedule(wg, ms.ORDER_XCD_DM_MAJOR)
         outs = []
         for c in range(calls):
